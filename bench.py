@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Benchmark of the neo-Hookean Tet P1 assembly on one MI355X.
+
+One step = one Energy::eval(x, gradient=True, hessian=Exact) of the 64^3 Kuhn-grid workload (1,572,864 tets,
+274,625 vertices, 36,168,777 Hessian values; E = 1e5, nu = 0.45, 20% stretch in x plus U(+-0.02 h) noise): energy,
+gradient and the assembled CSR Hessian values, with inputs and outputs resident in HBM.  `value` is tets/s of the
+whole step (kernel launches, the status/energy read-back and the host synchronisation included).
+
+Also reported: the assembly kernel against the HBM roofline (HIP events around the kernel on its stream), the
+Projected-Hessian step, and the oracle (C++ CPU restatement of the reference algorithm) timed on this host's
+cores on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "tets/s neo-Hookean E/grad/Hess+CSR assembly; batched & backward vs plain"
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, vendor figure
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector, vendor figure
+# Algorithmic (compulsory) bytes per assembly: 176 B per tet (connectivity 16, Dm^-1 72, w 8, (lambda, mu) 16,
+# 16 int32 block slots 64), 48 B per vertex (x read, g write), 8 B per Hessian value written.
+BYTES_PER_TET, BYTES_PER_VERTEX, BYTES_PER_NNZ = 176, 48, 8
+
+
+def algorithmic_bytes(T, V, nnz):
+    return BYTES_PER_TET * T + BYTES_PER_VERTEX * V + BYTES_PER_NNZ * nnz
+
+
+def read_traffic(path):
+    """HBM bytes per assembly launch from a committed rocprofv3 --pmc summary, or None."""
+    if not path.exists():
+        return None
+    text = path.read_text()
+    m = re.search(r"traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(w, nthreads, repeats):
+    """The oracle (oracle/liblpg_oracle.so) on the same 64^3 workload, E+g+H(Exact), median of `repeats`."""
+    from oracle import oracle
+
+    oracle.build()
+    ref = oracle.OracleEnergy(w.num_vertices, w.tets, w.rest, w.theta, nthreads=nthreads)
+    ref.eval(w.u, True, "exact", nthreads=nthreads)  # warm-up
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        ref.eval(w.u, True, "exact", nthreads=nthreads)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return {"value": w.tets.shape[0] / med, "unit": "tets/s", "cores": nthreads, "kind": "port",
+            "sample": f"oracle E+g+H(Exact)+CSR on the full 64^3 workload, median of {repeats} evals after 1 "
+                      f"warm-up, {nthreads} threads ({med * 1e3:.1f} ms per eval)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-repeats", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the sharded design is described "
+              "in DESIGN.md.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from litepgo_amd import VolumetricFemEnergy, workloads
+
+    assert torch.cuda.is_available(), "no HIP device"
+    w = workloads.plain_large()
+    T, V = w.tets.shape[0], w.num_vertices
+    dev = VolumetricFemEnergy(V, w.tets, w.rest, w.theta)
+    nnz = dev.nnz
+    x = torch.from_numpy(w.u).cuda()
+    g = torch.empty(dev.num_dofs, dtype=torch.float64, device="cuda")
+    H = torch.empty(nnz, dtype=torch.float64, device="cuda")
+
+    def step(kind):
+        return dev.eval_device(x, True, kind, out_g=g, out_H=H)
+
+    for _ in range(args.warmup):
+        step("exact")
+    dev.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step("exact")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_kernels, evals = dev.kernel_times()
+    ms_per_step = elapsed / args.steps * 1e3
+    assemble_ms = ms_kernels[1] / evals
+    B = algorithmic_bytes(T, V, nnz)
+    achieved = B / (assemble_ms * 1e-3) / 1e9
+
+    # Projected Hessian, same workload.
+    for _ in range(2):
+        step("projected")
+    dev.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    psteps = max(5, args.steps // 2)
+    for _ in range(psteps):
+        step("projected")
+    torch.cuda.synchronize()
+    p_elapsed = time.perf_counter() - t0
+    p_ms, p_evals = dev.kernel_times()
+    indefinite = dev.last_indefinite_count
+
+    result = {
+        "metric": METRIC,
+        "value": T / (ms_per_step * 1e-3),
+        "unit": "tets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "plain 64^3 Kuhn grid, E+g+H(Exact)+CSR assembly, E=1e5 nu=0.45, 20% x-stretch "
+                               "+ U(+-0.02h)", "tets": T, "vertices": V, "hessian_nnz": nnz,
+                   "launch": list(dev.get_launch())},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": read_traffic(ROOT / "profiles" / "assemble_exact_64cube_pmc.txt"),
+                     "kernel": "k_assemble (Exact, gradient)", "kernel_ms": assemble_ms,
+                     "algorithmic_bytes": B},
+        "projected": {"tets_per_s": T / (p_elapsed / psteps), "ms_per_step": p_elapsed / psteps * 1e3,
+                      "assemble_ms": p_ms[1] / p_evals, "projection_ms": p_ms[0] / p_evals,
+                      "indefinite_tets": indefinite, "indefinite_fraction": indefinite / T},
+    }
+    if not args.no_cpu:
+        nthreads = min(16, len(os.sched_getaffinity(0)))
+        result["cpu_baseline"] = cpu_baseline(w, nthreads, args.cpu_repeats)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,140 @@
+// Internal declarations shared by the host setup, the kernels and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace lpg {
+
+// Error classes of the reference (std::invalid_argument, std::domain_error, std::logic_error).
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+enum Code { kOk = 0, kInvalid = 1, kDomain = 2, kLogic = 3, kBusy = 4, kDevice = 5 };
+
+[[noreturn]] inline void fail(int code, const std::string& msg) {
+    throw Error(code, msg);
+}
+
+// Device status bits, OR-ed by kernels and mapped to a Code by the host.
+enum StatusBits : unsigned {
+    kStatusNonFiniteX = 1u,     // x has a non-finite entry                      -> invalid_argument
+    kStatusBadTheta = 2u,       // lambda < 0, mu <= 0 or non-finite             -> invalid_argument
+    kStatusDetF = 4u,           // det F <= 0 or non-finite kinematics           -> domain_error
+    kStatusNonFiniteOut = 8u,   // non-finite element or assembled outputs       -> domain_error
+    kStatusProjection = 16u,    // projection failed to converge / non-finite    -> domain_error
+};
+
+// One tet record: 16 doubles = 128 bytes, 128-byte aligned, so a record is exactly one cache line.
+//   [0..8]   Dm^-1 row-major (row r = G(r + 1, :), G(0, :) = -sum of rows)
+//   [9]      w = det(Dm) / 6
+//   [10..11] lambda, mu (snapshot theta)
+//   [12..13] the 4 vertex ids as int32
+//   [14..15] unused
+constexpr int kRecDoubles = 16;
+
+// Incidence record of (vertex v, tet e): the tet id and, for each local slot b of e, the position of vertex
+// e[b] in v's sorted neighbour list (the column block of H in v's rows).  Narrow form packs 4 x uint8.
+struct IncNarrow {
+    int32_t tet;
+    uint32_t colpos; // 4 x 8 bits
+};
+struct IncWide {
+    int32_t tet;
+    uint32_t colpos01; // 2 x 16 bits
+    uint32_t colpos23;
+    uint32_t pad;
+};
+
+// Host-side result of construction (everything the kernels read, plus the exported pattern).
+struct MeshSetup {
+    int32_t num_vertices = 0;
+    int32_t num_tets = 0;
+    int32_t num_dofs = 0;
+    int32_t nnz = 0;
+    int32_t max_degree = 0;          // max |neighbours incl. self| over vertices
+    bool wide = false;               // max_degree > 255
+    bool theta_valid = true;
+    std::vector<int32_t> row_ptr;    // num_dofs + 1
+    std::vector<int32_t> col_idx;    // nnz
+    std::vector<int32_t> nbr_ptr;    // num_vertices + 1
+    std::vector<int32_t> nbr;        // sorted vertex neighbours incl. self
+    std::vector<int32_t> inc_ptr;    // num_vertices + 1
+    std::vector<int32_t> inc_tet;    // 4 T, element order per vertex
+    std::vector<uint16_t> inc_colpos; // 4 T x 4
+    std::vector<double> rec;         // T x kRecDoubles
+};
+
+// Builds MeshSetup; throws Error(kInvalid, ...) with the reference's messages on invalid input.
+void build_mesh_setup(MeshSetup& s, int32_t nv, int32_t nt, const int32_t* tets, const double* rest,
+                      const double* theta, const double* Fp, int nthreads);
+
+// Reference-convention slot map from the setup (offsets[144 e + i + 12 j]).
+void build_slot_map(const MeshSetup& s, int32_t* offsets);
+
+// Vertex tiles: contiguous vertex ranges handled by one workgroup.  tile_vbeg has tiles + 1 entries.
+struct TileConfig {
+    int vertices_per_tile;    // upper bound on vertices in a tile
+    int lds_budget_doubles;   // upper bound on the H segment staged in LDS
+};
+void build_tiles(const MeshSetup& s, const TileConfig& cfg, std::vector<int32_t>& tile_vbeg,
+                 int32_t& max_segment, std::vector<int32_t>* oversized);
+
+// ------------------------------------------------------------------------------------------------------------
+// Kernel parameters and launchers (lpg_kernels.hip).
+// ------------------------------------------------------------------------------------------------------------
+struct DeviceMesh {
+    const double* rec = nullptr;
+    const int32_t* inc_ptr = nullptr;
+    const void* inc = nullptr; // IncNarrow* or IncWide*
+    const int32_t* row_ptr = nullptr;
+    int32_t num_vertices = 0;
+    int32_t num_tets = 0;
+    bool wide = false;
+};
+
+struct EvalArgs {
+    const double* x = nullptr;
+    const double* theta = nullptr; // optional override, T x 2
+    double* g = nullptr;
+    double* H = nullptr;
+    int want_g = 0;
+    int kind = -1;
+    // scratch
+    double* partial_E = nullptr; // one per tile
+    double* energy = nullptr;    // 1 double (device)
+    unsigned* status = nullptr;  // 1 word (device)
+    unsigned* counter = nullptr; // indefinite count (device)
+    int32_t* proj_slot = nullptr; // T
+    int32_t* proj_list = nullptr; // T
+    double* proj_M = nullptr;    // T x 45
+};
+
+struct LaunchConfig {
+    int lanes_per_vertex = 1; // 1, 2 or 4
+    int waves = 1;            // waves per workgroup
+};
+
+void launch_clear(unsigned* status, unsigned* counter, hipStream_t stream);
+// Processes tiles tile_ids[0..num_tiles) (or 0..num_tiles-1 when tile_ids is null).  global_acc accumulates in
+// the H output itself instead of LDS (for tiles whose segment exceeds the LDS budget).
+void launch_assemble(const DeviceMesh& m, const EvalArgs& a, const LaunchConfig& lc, const int32_t* tile_vbeg,
+                     const int32_t* tile_ids, int num_tiles, int lds_doubles, bool global_acc, hipStream_t stream);
+void launch_energy_reduce(const double* partial, int n, double* out, unsigned* status, hipStream_t stream);
+void launch_project_prepass(const DeviceMesh& m, const EvalArgs& a, hipStream_t stream);
+void launch_project_jacobi(const DeviceMesh& m, const EvalArgs& a, hipStream_t stream);
+void launch_dE_dtheta(const DeviceMesh& m, const double* x, const double* theta, double* out, unsigned* status,
+                      hipStream_t stream);
+void launch_force_vjp_theta(const DeviceMesh& m, const double* x, const double* theta, const double* w,
+                            double* out, unsigned* status, hipStream_t stream);
+void launch_force_jvp_theta(const DeviceMesh& m, const double* x, const double* theta, const double* dtheta,
+                            double* out, unsigned* status, hipStream_t stream);
+void launch_hvp(const DeviceMesh& m, const double* x, const double* theta, const double* v, double* out,
+                unsigned* status, hipStream_t stream);
+
+} // namespace lpg

@@ -1,0 +1,257 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and the golden fixtures, on an MI355X.
+
+Bar: CSR pattern and slot map bit-exact; energy, gradient and Hessian values within 1e-12 relative max-norm
+(|a - b|_inf / |b|_inf per output array) of the oracle, for Exact and Projected Hessians.
+"""
+import hashlib
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from litepgo_amd import VolumetricFemEnergy, kuhn_grid, lame_parameters, workloads
+from litepgo_amd import _native
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    assert _native.LIB_PATH.exists(), "liblitepgo_fem.so is not built"
+    return torch
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    scale = np.abs(b).max() if b.size else 0.0
+    if scale == 0.0:
+        return float(np.abs(a).max()) if a.size else 0.0
+    return float(np.abs(a - b).max() / scale)
+
+
+def check_against_oracle(dev, ref, x, kinds=("exact", "projected"), theta=None, tol=TOL):
+    for kind in kinds:
+        print(f"  kind={kind}", flush=True)
+        v, g, H = dev.eval_values(x, True, kind)
+        vr, gr, Hr = ref.eval(x, True, kind, theta=theta)
+        assert abs(v - vr) <= tol * max(abs(vr), 1e-300), (kind, v, vr)
+        assert rel(g, gr) <= tol, (kind, rel(g, gr))
+        assert rel(H, Hr) <= tol, (kind, rel(H, Hr))
+
+
+def make(oracle_mod, X, tets, theta):
+    return VolumetricFemEnergy(len(X), tets, X, theta), oracle_mod.OracleEnergy(len(X), tets, X, theta)
+
+
+def test_two_tet_fixture(oracle_mod):
+    d = np.load(GOLD / "two_tet.npz")
+    dev = VolumetricFemEnergy(5, d["tets"], d["X"], d["theta"])
+    assert np.array_equal(dev.pattern()[0], d["outer"]) and np.array_equal(dev.pattern()[1], d["inner"])
+    assert np.array_equal(dev.slot_map(), d["offsets"])
+    for s in range(3):
+        v, g, H = dev.eval_values(d["x"][s], True, "exact")
+        _, _, Hp = dev.eval_values(d["x"][s], False, "projected")
+        assert abs(v - d["value_exact"][s]) <= TOL * max(abs(d["value_exact"][s]), 1e-300)
+        assert rel(g, d["grad"][s]) <= TOL
+        assert rel(H, d["H_exact"][s]) <= TOL
+        assert rel(Hp, d["H_projected"][s]) <= TOL
+
+
+def test_grid2_fixture_and_oracle(oracle_mod):
+    d = np.load(GOLD / "grid2.npz")
+    dev, ref = make(oracle_mod, d["X"], d["tets"], d["theta"])
+    for s in range(2):
+        print(f"grid2 state {s}", flush=True)
+        v, g, H = dev.eval_values(d["x"][s], True, "exact")
+        _, _, Hp = dev.eval_values(d["x"][s], False, "projected")
+        assert abs(v - d["value_exact"][s]) <= TOL * abs(d["value_exact"][s])
+        assert rel(g, d["grad"][s]) <= TOL
+        assert rel(H, d["H_exact"][s]) <= TOL
+        assert rel(Hp, d["H_projected"][s]) <= TOL
+        check_against_oracle(dev, ref, d["x"][s])
+    assert dev.last_indefinite_count > 0  # the compressed state has indefinite elements
+
+
+def test_plain_small_config(oracle_mod):
+    """10^3 Kuhn grid, E = 1e5, nu = 0.4, u ~ U(+-0.05 h): the reference's CPU-sized configuration."""
+    w = workloads.plain_small()
+    dev, ref = make(oracle_mod, w.rest, w.tets, w.theta)
+    check_against_oracle(dev, ref, w.u)
+    d = np.load(GOLD / "grid10.npz")
+    v, g, H = dev.eval_values(d["x"], True, "exact")
+    _, _, Hp = dev.eval_values(d["x"], False, "projected")
+    s = d["sample"]
+    assert abs(v - d["value_exact"]) <= TOL * abs(d["value_exact"])
+    assert rel(g, d["grad"]) <= TOL
+    assert np.abs(H[s] - d["H_exact_sample"]).max() <= TOL * d["H_exact_absmax"]
+    assert np.abs(Hp[s] - d["H_projected_sample"]).max() <= TOL * d["H_projected_absmax"]
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()  # noqa: E731
+    assert sha(dev.pattern()[0]) == str(d["outer_sha"]) and sha(dev.pattern()[1]) == str(d["inner_sha"])
+    assert sha(dev.slot_map()) == str(d["offsets_sha"])
+
+
+def test_device_tensor_interface_matches_host_interface(gpu):
+    torch = gpu
+    w = workloads.plain_small(seed=3)
+    dev = VolumetricFemEnergy(w.num_vertices, w.tets, w.rest, w.theta)
+    x = torch.from_numpy(w.u).cuda()
+    for kind in ("exact", "projected"):
+        v, g, H = dev.eval_device(x, True, kind)
+        vh, gh, Hh = dev.eval_values(w.u, True, kind)
+        assert v == vh
+        assert np.array_equal(g.cpu().numpy(), gh) and np.array_equal(H.cpu().numpy(), Hh)
+    v, g, H = dev.eval_device(x)
+    assert g is None and H is None and v == dev.eval_value(w.u)
+
+
+@pytest.mark.parametrize("cfg", [(1, 1, 64, 12000), (1, 1, 16, 6144), (2, 1, 32, 6144), (4, 1, 16, 6144),
+                                 (2, 2, 64, 12000), (4, 4, 64, 12000), (2, 1, 32, 200)])
+def test_launch_configurations_are_bitwise_identical(cfg):
+    """Every value is summed in element order by one lane whatever the launch shape, so results are bitwise equal;
+    the last configuration's tiny LDS budget sends every tile through the HBM-accumulation path."""
+    w = workloads.plain(6, 1e5, 0.4, 0.15, seed=5)
+    dev = VolumetricFemEnergy(w.num_vertices, w.tets, w.rest, w.theta)
+    base = [dev.eval_values(w.u, True, k) for k in ("exact", "projected")]
+    dev.set_launch(*cfg)
+    assert dev.get_launch() == cfg
+    for k, b in zip(("exact", "projected"), base):
+        v, g, H = dev.eval_values(w.u, True, k)
+        assert v == b[0] and np.array_equal(g, b[1]) and np.array_equal(H, b[2])
+
+
+def test_run_to_run_determinism():
+    w = workloads.plain(12, 1e5, 0.45, 0.1, stretch=0.1, seed=9)
+    dev = VolumetricFemEnergy(w.num_vertices, w.tets, w.rest, w.theta)
+    a = dev.eval_values(w.u, True, "projected")
+    b = dev.eval_values(w.u, True, "projected")
+    assert a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+
+
+def test_irregular_numbering_and_unused_vertex(oracle_mod):
+    rng = np.random.default_rng(21)
+    X, tets = kuhn_grid((3, 4, 5))
+    perm_v = rng.permutation(len(X))
+    inv = np.empty_like(perm_v)
+    inv[perm_v] = np.arange(len(X))
+    X = np.vstack([X[perm_v], [[7.0, 7.0, 7.0]]])
+    tets = inv[tets][rng.permutation(len(tets))].astype(np.int32)
+    theta = np.stack([rng.uniform(1e3, 1e5, len(tets)), rng.uniform(1e3, 1e5, len(tets))], 1)
+    dev, ref = make(oracle_mod, X, tets, theta)
+    x = rng.uniform(-0.04, 0.04, X.size)
+    x[-3:] = 0.5  # the unused vertex: zero gradient, empty rows
+    check_against_oracle(dev, ref, x)
+    v, g, H = dev.eval_values(x, True, "exact")
+    assert np.all(g[-3:] == 0.0)
+
+
+def test_high_degree_vertex_uses_wide_incidences(oracle_mod):
+    """A fan of 120 tets around one vertex: its degree (361) exceeds the 8-bit column-position form."""
+    rng = np.random.default_rng(4)
+    n = 120
+    X = [np.zeros(3)]
+    tets = []
+    for e in range(n):
+        base = rng.normal(size=3)
+        base /= np.linalg.norm(base)
+        pts = [base * 2 + rng.normal(scale=0.3, size=3) for _ in range(3)]
+        idx = list(range(len(X), len(X) + 3))
+        X.extend(pts)
+        t = [0] + idx
+        D = np.stack([X[t[k]] - X[t[0]] for k in (1, 2, 3)], 1)
+        if np.linalg.det(D) < 0:
+            t[1], t[2] = t[2], t[1]
+        tets.append(t)
+    X = np.array(X)
+    tets = np.array(tets, dtype=np.int32)
+    theta = np.tile([[1e4, 2e4]], (n, 1))
+    dev, ref = make(oracle_mod, X, tets, theta)
+    assert dev.max_degree == 3 * n + 1
+    x = rng.uniform(-0.01, 0.01, X.size)
+    check_against_oracle(dev, ref, x)
+
+
+def test_error_semantics_and_recovery():
+    w = workloads.plain(4, 1e5, 0.4, 0.05, seed=1)
+    dev = VolumetricFemEnergy(w.num_vertices, w.tets, w.rest, w.theta)
+    good = dev.eval_values(w.u, True, "exact")
+    x = w.u.copy()
+    x[5] = math.nan
+    with pytest.raises(ValueError, match="finite"):
+        dev.eval_values(x, True, "exact")
+    inverted = -2.0 * w.rest.reshape(-1)  # F = -I everywhere
+    with pytest.raises(_native.DomainError):
+        dev.eval_values(inverted, True, "exact")
+    huge = w.u.copy()
+    huge[0] = 1e155  # finite input whose density overflows
+    with pytest.raises(_native.DomainError):
+        dev.eval_values(huge)
+    bad = VolumetricFemEnergy(w.num_vertices, w.tets, w.rest, np.vstack([w.theta[:-1], [[-1.0, 1.0]]]))
+    with pytest.raises(ValueError, match="lambda"):
+        bad.eval_values(w.u)
+    with pytest.raises(ValueError, match="Hessian kind"):
+        dev.eval_values(w.u, False, 2)
+    again = dev.eval_values(w.u, True, "exact")  # the object stays usable
+    assert again[0] == good[0] and np.array_equal(again[1], good[1]) and np.array_equal(again[2], good[2])
+
+
+def test_theta_override_and_derivatives(oracle_mod, gpu):
+    torch = gpu
+    w, E, nu = workloads.differentiate(n=6, seed=2)
+    dev, ref = make(oracle_mod, w.rest, w.tets, w.theta)
+    x = torch.from_numpy(w.u).cuda()
+    theta2 = w.theta * np.random.default_rng(0).uniform(0.5, 1.5, w.theta.shape)
+    t2 = torch.from_numpy(theta2).cuda()
+    v, g, H = dev.eval_device(x, True, "exact", theta=t2)
+    vr, gr, Hr = ref.eval(w.u, True, "exact", theta=theta2)
+    assert abs(v - vr) <= TOL * abs(vr) and rel(g.cpu().numpy(), gr) <= TOL and rel(H.cpu().numpy(), Hr) <= TOL
+    # dE/dtheta
+    assert rel(dev.dE_dtheta_device(x).cpu().numpy(), ref.dE_dtheta(w.u)) <= TOL
+    # force VJP / JVP in theta
+    rng = np.random.default_rng(1)
+    wv = rng.normal(size=w.u.size)
+    assert rel(dev.force_vjp_theta_device(x, torch.from_numpy(wv).cuda()).cpu().numpy(),
+               ref.force_theta_vjp(w.u, wv)) <= TOL
+    dth = rng.normal(size=w.theta.shape)
+    assert rel(dev.force_jvp_theta_device(x, torch.from_numpy(dth).cuda()).cpu().numpy(),
+               ref.force_theta_jvp(w.u, dth)) <= TOL
+    # Hessian-vector product against the oracle's assembled Exact Hessian
+    import scipy.sparse as sp
+
+    _, _, Hr = ref.eval(w.u, False, "exact")
+    outer, inner = ref.pattern()
+    Hm = sp.csc_matrix((Hr, inner, outer), shape=(ref.num_dofs, ref.num_dofs))
+    vv = rng.normal(size=w.u.size)
+    assert rel(dev.hvp_device(x, torch.from_numpy(vv).cuda()).cpu().numpy(), Hm @ vv) <= TOL
+
+
+def test_plain_large_config_full_size(oracle_mod):
+    """64^3 Kuhn grid (1,572,864 tets), E = 1e5, nu = 0.45, 20% stretch + U(+-0.02 h): full-size parity with the
+    oracle plus size-independent invariants (translation invariance, zero net force)."""
+    w = workloads.plain_large()
+    print("building device energy", flush=True)
+    dev = VolumetricFemEnergy(w.num_vertices, w.tets, w.rest, w.theta)
+    print("building oracle", flush=True)
+    ref = oracle_mod.OracleEnergy(w.num_vertices, w.tets, w.rest, w.theta)
+    assert dev.nnz == ref.nnz == 36168777
+    r, c = dev.pattern()
+    ro, co = ref.pattern()
+    assert np.array_equal(r, ro) and np.array_equal(c, co)
+    print("comparing values", flush=True)
+    check_against_oracle(dev, ref, w.u)
+    v, g, H = dev.eval_values(w.u, True, "exact")
+    # a rigid translation is in the null space of every element Hessian; the net force vanishes
+    import scipy.sparse as sp
+
+    Hm = sp.csr_matrix((H, c, r), shape=(dev.num_dofs, dev.num_dofs))
+    for axis in range(3):
+        t = np.zeros(dev.num_dofs)
+        t[axis::3] = 1.0
+        assert np.abs(Hm @ t).max() <= 1e-12 * np.abs(H).max() * 30
+    assert abs(g[0::3].sum()) <= 1e-10 * np.abs(g).sum()

@@ -1,0 +1,85 @@
+"""Host-side logic of the C ABI that needs no GPU: validation at construction, the CSR pattern and the
+reference-convention slot map (bit-exact against the oracle), sizes, and loud failure without a device."""
+import math
+
+import numpy as np
+import pytest
+
+from litepgo_amd import VolumetricFemEnergy, kuhn_grid, lame_parameters
+from litepgo_amd import _native
+
+
+def energy_pair(oracle_mod, X, tets, theta):
+    return VolumetricFemEnergy(len(X), tets, X, theta), oracle_mod.OracleEnergy(len(X), tets, X, theta)
+
+
+@pytest.mark.parametrize("dims", [1, 3, (2, 3, 4), 10])
+def test_pattern_and_slot_map_match_oracle(oracle_mod, dims):
+    X, tets = kuhn_grid(dims)
+    theta = np.tile([[2.0, 3.0]], (len(tets), 1))
+    dev, ref = energy_pair(oracle_mod, X, tets, theta)
+    assert dev.num_dofs == ref.num_dofs and dev.nnz == ref.nnz
+    r, c = dev.pattern()
+    ro, co = ref.pattern()
+    assert np.array_equal(r, ro) and np.array_equal(c, co)
+    assert np.array_equal(dev.slot_map(), ref.offsets())
+
+
+def test_irregular_mesh_pattern(oracle_mod):
+    # shuffled element order, an unused vertex, shuffled vertex numbering
+    rng = np.random.default_rng(11)
+    X, tets = kuhn_grid(3)
+    perm_v = rng.permutation(len(X))
+    inv = np.empty_like(perm_v)
+    inv[perm_v] = np.arange(len(X))
+    X = X[perm_v]
+    tets = inv[tets][rng.permutation(len(tets))].astype(np.int32)
+    X = np.vstack([X, [[9.0, 9.0, 9.0]]])
+    theta = np.tile([[2.0, 3.0]], (len(tets), 1))
+    dev, ref = energy_pair(oracle_mod, X, tets, theta)
+    assert np.array_equal(dev.pattern()[0], ref.pattern()[0])
+    assert np.array_equal(dev.pattern()[1], ref.pattern()[1])
+    assert np.array_equal(dev.slot_map(), ref.offsets())
+
+
+def test_construction_errors_follow_the_reference():
+    X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0.3, 0.2, -2]], dtype=np.float64)
+    T = np.array([[0, 1, 2, 3], [0, 2, 1, 4]], dtype=np.int32)
+    th = np.array([[2.0, 3.0]] * 2)
+    for scale in (0.0, -1.0, 1e200):
+        with pytest.raises(ValueError):
+            VolumetricFemEnergy(5, T, scale * X, th)
+    for bad in (math.nan, math.inf):
+        Xb = X.copy()
+        Xb[0, 0] = bad
+        with pytest.raises(ValueError):
+            VolumetricFemEnergy(5, T, Xb, th)
+    with pytest.raises(ValueError, match="distinct"):
+        VolumetricFemEnergy(5, np.array([[0, 1, 1, 3]]), X, th[:1])
+    with pytest.raises(ValueError, match="outside"):
+        VolumetricFemEnergy(5, np.array([[0, 1, 2, 7]]), X, th[:1])
+    with pytest.raises(ValueError, match="batch size"):
+        VolumetricFemEnergy(5, T, X, th, batch_size=0)
+    with pytest.raises(ValueError, match="Fp = I"):
+        VolumetricFemEnergy(5, T, X, th, Fp=np.stack([2 * np.eye(3)] * 2))
+    # identity Fp is accepted; invalid theta is accepted at construction (NeoHookean checks it when evaluating)
+    VolumetricFemEnergy(5, T, X, th, Fp=np.stack([np.eye(3)] * 2))
+    VolumetricFemEnergy(5, T, X, np.array([[2.0, 3.0], [-1.0, 3.0]]))
+
+
+def test_sizes_of_the_benchmark_meshes():
+    X, tets = kuhn_grid(10)
+    lam, mu = lame_parameters(1e5, 0.4)
+    dev = VolumetricFemEnergy(len(X), tets, X, np.array([float(lam), float(mu)]))
+    assert (dev.num_tets, dev.num_dofs, dev.nnz, dev.max_degree) == (6000, 3993, 154719, 15)
+
+
+def test_no_cpu_fallback_without_a_gpu():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    X, tets = kuhn_grid(1)
+    dev = VolumetricFemEnergy(len(X), tets, X, np.array([2.0, 3.0]))
+    with pytest.raises(_native.DeviceError):
+        dev.eval(np.zeros(dev.num_dofs), True, "exact")
