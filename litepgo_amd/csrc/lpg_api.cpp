@@ -1,166 +1,100 @@
-// C ABI (include/litepgo_fem.h): handle lifetime, validation, device buffers, launch sequencing, status mapping.
+// C ABI of the plain energy (include/litepgo_fem.h): handle lifetime, validation, device buffers, launch
+// sequencing and status mapping.
 #include "litepgo_fem.h"
 #include "litepgo_fem_ext.h"
-#include "lpg_internal.hpp"
+#include "lpg_handle.hpp"
 
 #include <algorithm>
-#include <atomic>
 #include <cmath>
 #include <cstring>
-#include <mutex>
-#include <string>
+#include <memory>
 #include <thread>
-#include <vector>
 
-namespace {
+namespace lpg {
 
 thread_local std::string g_last_error;
 
-struct DeviceBuffer {
-    void* ptr = nullptr;
-    size_t bytes = 0;
-    void ensure(size_t want) {
-        if (want <= bytes && ptr)
-            return;
-        release();
-        if (want == 0)
-            return;
-        if (hipMalloc(&ptr, want) != hipSuccess) {
-            ptr = nullptr;
-            bytes = 0;
-            lpg::fail(lpg::kDevice, "hipMalloc of " + std::to_string(want) + " bytes failed");
-        }
-        bytes = want;
-    }
-    void release() {
-        if (ptr)
-            (void)hipFree(ptr);
+void hip_check(hipError_t err, const char* what) {
+    if (err != hipSuccess)
+        fail(kDevice, std::string(what) + ": " + hipGetErrorString(err));
+}
+
+void DeviceBuffer::ensure(size_t want) {
+    if (want <= bytes && ptr)
+        return;
+    release();
+    if (want == 0)
+        return;
+    if (hipMalloc(&ptr, want) != hipSuccess) {
         ptr = nullptr;
         bytes = 0;
+        fail(kDevice, "hipMalloc of " + std::to_string(want) + " bytes failed");
     }
-    template <class T> T* as() const {
-        return static_cast<T*>(ptr);
-    }
-};
+    bytes = want;
+}
+
+void DeviceBuffer::release() {
+    if (ptr)
+        (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
 
 void upload(DeviceBuffer& buf, const void* src, size_t bytes) {
     buf.ensure(bytes);
     if (bytes && hipMemcpy(buf.ptr, src, bytes, hipMemcpyHostToDevice) != hipSuccess)
-        lpg::fail(lpg::kDevice, "hipMemcpy to device failed");
+        fail(kDevice, "hipMemcpy to device failed");
 }
 
-void hip_check(hipError_t err, const char* what) {
-    if (err != hipSuccess)
-        lpg::fail(lpg::kDevice, std::string(what) + ": " + hipGetErrorString(err));
+BusyScope::BusyScope(std::atomic<bool>& f) : flag(f) {
+    bool expected = false;
+    if (!flag.compare_exchange_strong(expected, true))
+        fail(kBusy, "MeshEnergy: overlapping evaluation of one energy object");
+}
+BusyScope::~BusyScope() {
+    flag.store(false);
 }
 
-struct TilePlan {
-    lpg::LaunchConfig lc;
-    int vertices_per_tile = 0;
-    int lds_budget = 0;
-    std::vector<int32_t> tile_vbeg;
-    std::vector<int32_t> normal_ids, oversized_ids;
-    int32_t max_segment = 0;
-    DeviceBuffer d_tile_vbeg, d_normal_ids, d_oversized_ids, d_partial;
-    int num_tiles() const {
-        return static_cast<int>(tile_vbeg.size()) - 1;
-    }
-};
+} // namespace lpg
 
-} // namespace
-
-struct lpg_tet_energy {
-    lpg::MeshSetup setup;
-    int device = 0;
-    hipStream_t own_stream = nullptr;
-    DeviceBuffer d_rec, d_inc_ptr, d_inc, d_row_ptr;
-    DeviceBuffer d_scalars; // energy (double) + status + counter
-    DeviceBuffer d_proj_slot, d_proj_list, d_proj_M;
-    DeviceBuffer d_x, d_g, d_H; // staging for the host-pointer API
-    DeviceBuffer d_aux;         // scratch for derivative entry points
-    TilePlan plan;
-    std::atomic<bool> busy{false};
-    int32_t last_indefinite = 0;
-    bool device_ready = false;
-    void* pinned = nullptr; // energy + status + counter read-back
-    bool timing = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    double time_ms[4] = {0, 0, 0, 0};
-    int32_t timed_evals = 0;
-
-    lpg::DeviceMesh mesh() const {
-        lpg::DeviceMesh m;
-        m.rec = d_rec.as<double>();
-        m.inc_ptr = d_inc_ptr.as<int32_t>();
-        m.inc = d_inc.ptr;
-        m.row_ptr = d_row_ptr.as<int32_t>();
-        m.num_vertices = setup.num_vertices;
-        m.num_tets = setup.num_tets;
-        m.wide = setup.wide;
-        return m;
-    }
-    double* d_energy() const {
-        return d_scalars.as<double>();
-    }
-    unsigned* d_status() const {
-        return reinterpret_cast<unsigned*>(d_scalars.as<char>() + 8);
-    }
-    unsigned* d_counter() const {
-        return reinterpret_cast<unsigned*>(d_scalars.as<char>() + 12);
-    }
-    ~lpg_tet_energy() {
-        for (DeviceBuffer* b : {&d_rec, &d_inc_ptr, &d_inc, &d_row_ptr, &d_scalars, &d_proj_slot, &d_proj_list,
-                                &d_proj_M, &d_x, &d_g, &d_H, &d_aux, &plan.d_tile_vbeg, &plan.d_normal_ids,
-                                &plan.d_oversized_ids, &plan.d_partial})
-            b->release();
-        if (pinned)
-            (void)hipHostFree(pinned);
-        for (hipEvent_t e : ev)
-            if (e)
-                (void)hipEventDestroy(e);
-        if (own_stream)
-            (void)hipStreamDestroy(own_stream);
-    }
-};
-
-namespace {
-
-struct BusyScope {
-    std::atomic<bool>& flag;
-    explicit BusyScope(std::atomic<bool>& f) : flag(f) {
-        bool expected = false;
-        if (!flag.compare_exchange_strong(expected, true))
-            lpg::fail(lpg::kBusy, "MeshEnergy: overlapping evaluation of one energy object");
-    }
-    ~BusyScope() {
-        flag.store(false);
-    }
-};
-
-template <class F> lpg_status guarded(F&& body) {
-    try {
-        body();
-        return LPG_OK;
-    } catch (const lpg::Error& e) {
-        g_last_error = e.what();
-        return static_cast<lpg_status>(e.code);
-    } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
-        return LPG_DEVICE_ERROR;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return LPG_LOGIC_ERROR;
-    }
+lpg::DeviceMesh lpg_tet_energy::mesh() const {
+    lpg::DeviceMesh m;
+    m.rec = d_rec.as<double>();
+    m.inc_ptr = d_inc_ptr.as<int32_t>();
+    m.inc = d_inc.ptr;
+    m.row_ptr = d_row_ptr.as<int32_t>();
+    m.num_vertices = setup.num_vertices;
+    m.num_tets = setup.num_tets;
+    m.wide = setup.wide;
+    return m;
 }
 
-void set_plan(lpg_tet_energy* h, const lpg::LaunchConfig& lc, int vertices_per_tile, int lds_budget) {
+lpg_tet_energy::~lpg_tet_energy() {
+    for (lpg::DeviceBuffer* b : {&d_rec, &d_inc_ptr, &d_inc, &d_row_ptr, &d_scalars, &d_vertex_E, &d_chunk_begin,
+                                 &d_chunk_sum, &d_seg_chunk, &d_proj_slot, &d_proj_list, &d_proj_M, &d_x, &d_g, &d_H,
+                                 &plan.d_tile_vbeg, &plan.d_normal_ids, &plan.d_oversized_ids})
+        b->release();
+    if (pinned)
+        (void)hipHostFree(pinned);
+    for (hipEvent_t e : ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    if (own_stream)
+        (void)hipStreamDestroy(own_stream);
+}
+
+namespace lpg {
+
+void set_plan(lpg_tet_energy* h, const LaunchConfig& lc, int vertices_per_tile, int lds_budget,
+              int persistent_grid) {
     TilePlan& p = h->plan;
     p.lc = lc;
     p.vertices_per_tile = vertices_per_tile;
     p.lds_budget = lds_budget;
+    p.persistent_grid = persistent_grid;
     std::vector<int32_t> oversized;
-    lpg::build_tiles(h->setup, lpg::TileConfig{vertices_per_tile, lds_budget}, p.tile_vbeg, p.max_segment,
-                     &oversized);
+    build_tiles(h->setup, TileConfig{vertices_per_tile, lds_budget}, p.tile_vbeg, p.max_segment, &oversized,
+                &h->seg_vbeg);
     p.oversized_ids = oversized;
     p.normal_ids.clear();
     if (!oversized.empty()) {
@@ -174,35 +108,6 @@ void set_plan(lpg_tet_energy* h, const lpg::LaunchConfig& lc, int vertices_per_t
     upload(p.d_tile_vbeg, p.tile_vbeg.data(), p.tile_vbeg.size() * sizeof(int32_t));
     upload(p.d_normal_ids, p.normal_ids.data(), p.normal_ids.size() * sizeof(int32_t));
     upload(p.d_oversized_ids, p.oversized_ids.data(), p.oversized_ids.size() * sizeof(int32_t));
-    p.d_partial.ensure(std::max<size_t>(1, p.num_tiles()) * sizeof(double));
-}
-
-hipStream_t stream_of(lpg_tet_energy* h, void* s) {
-    return s ? static_cast<hipStream_t>(s) : h->own_stream;
-}
-
-// Reads energy, status and counter back and maps the status to the reference's exception classes.
-double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter) {
-    hip_check(hipMemcpyAsync(h->pinned, h->d_scalars.ptr, 16, hipMemcpyDeviceToHost, stream), "read-back");
-    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    double value;
-    unsigned status, counter;
-    std::memcpy(&value, h->pinned, 8);
-    std::memcpy(&status, static_cast<char*>(h->pinned) + 8, 4);
-    std::memcpy(&counter, static_cast<char*>(h->pinned) + 12, 4);
-    if (want_counter)
-        h->last_indefinite = static_cast<int32_t>(counter);
-    if (status & lpg::kStatusNonFiniteX)
-        lpg::fail(lpg::kInvalid, "Energy::eval: x must have num_dofs finite entries");
-    if (status & lpg::kStatusBadTheta)
-        lpg::fail(lpg::kInvalid, "NeoHookean requires finite lambda >= 0 and mu > 0");
-    if (status & lpg::kStatusDetF)
-        lpg::fail(lpg::kDomain, "NeoHookean requires finite det(Fe) > 0");
-    if (status & lpg::kStatusProjection)
-        lpg::fail(lpg::kDomain, "MeshEnergy: non-finite projected Hessian");
-    if (status & lpg::kStatusNonFiniteOut)
-        lpg::fail(lpg::kDomain, "VolumetricElement3D: non-finite integrated outputs");
-    return value;
 }
 
 // Device state is created on the first call that needs the GPU, so construction, the pattern and the slot map
@@ -223,60 +128,92 @@ void ensure_device(lpg_tet_energy* h) {
     upload(h->d_row_ptr, s.row_ptr.data(), s.row_ptr.size() * sizeof(int32_t));
     const size_t ninc = s.inc_tet.size();
     if (!s.wide) {
-        std::vector<lpg::IncNarrow> inc(ninc);
+        std::vector<IncNarrow> inc(ninc);
         for (size_t k = 0; k < ninc; ++k) {
             uint32_t packed = 0;
             for (int b = 0; b < 4; ++b)
                 packed |= static_cast<uint32_t>(s.inc_colpos[4 * k + b]) << (8 * b);
             inc[k] = {s.inc_tet[k], packed};
         }
-        upload(h->d_inc, inc.data(), inc.size() * sizeof(lpg::IncNarrow));
+        upload(h->d_inc, inc.data(), inc.size() * sizeof(IncNarrow));
     } else {
-        std::vector<lpg::IncWide> inc(ninc);
+        std::vector<IncWide> inc(ninc);
         for (size_t k = 0; k < ninc; ++k) {
             inc[k].tet = s.inc_tet[k];
             inc[k].colpos01 = s.inc_colpos[4 * k] | (static_cast<uint32_t>(s.inc_colpos[4 * k + 1]) << 16);
             inc[k].colpos23 = s.inc_colpos[4 * k + 2] | (static_cast<uint32_t>(s.inc_colpos[4 * k + 3]) << 16);
             inc[k].pad = 0;
         }
-        upload(h->d_inc, inc.data(), inc.size() * sizeof(lpg::IncWide));
+        upload(h->d_inc, inc.data(), inc.size() * sizeof(IncWide));
     }
     h->d_scalars.ensure(64);
+    if (h->seg_vbeg.empty())
+        h->seg_vbeg = {0, s.num_vertices};
+    build_energy_chunks(h->seg_vbeg, h->chunk_begin, h->seg_chunk);
+    upload(h->d_chunk_begin, h->chunk_begin.data(), h->chunk_begin.size() * sizeof(int32_t));
+    upload(h->d_seg_chunk, h->seg_chunk.data(), h->seg_chunk.size() * sizeof(int32_t));
+    h->d_chunk_sum.ensure(std::max<size_t>(1, h->chunk_begin.size()) * sizeof(double));
+    h->d_vertex_E.ensure(std::max<size_t>(1, s.num_vertices) * sizeof(double));
     if (h->plan.tile_vbeg.empty()) {
         // Default launch: 2 lanes per vertex, one wave per workgroup, 32 vertices per tile, segments up to
-        // 6144 doubles (48 KiB) staged in LDS.
-        lpg::LaunchConfig lc;
+        // 6144 doubles (48 KiB) staged in LDS, one workgroup per tile.
+        LaunchConfig lc;
         lc.lanes_per_vertex = 2;
         lc.waves = 1;
-        set_plan(h, lc, 32, 6144);
+        set_plan(h, lc, 32, 6144, 0);
     }
     h->device_ready = true;
 }
 
-void eval_device(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* value,
-                 double* g, double* H, hipStream_t stream) {
+void raise_status(unsigned status) {
+    if (status & kStatusNonFiniteX)
+        fail(kInvalid, "Energy::eval: x must have num_dofs finite entries");
+    if (status & kStatusBadTheta)
+        fail(kInvalid, "NeoHookean requires finite lambda >= 0 and mu > 0");
+    if (status & kStatusDetF)
+        fail(kDomain, "NeoHookean requires finite det(Fe) > 0");
+    if (status & kStatusProjection)
+        fail(kDomain, "MeshEnergy: non-finite projected Hessian");
+    if (status & kStatusNonFiniteOut)
+        fail(kDomain, "VolumetricElement3D: non-finite integrated outputs");
+}
+
+double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter) {
+    hip_check(hipMemcpyAsync(h->pinned, h->d_scalars.ptr, 16, hipMemcpyDeviceToHost, stream), "read-back");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    double value;
+    unsigned status, counter;
+    std::memcpy(&value, h->pinned, 8);
+    std::memcpy(&status, static_cast<char*>(h->pinned) + 8, 4);
+    std::memcpy(&counter, static_cast<char*>(h->pinned) + 12, 4);
+    if (want_counter)
+        h->last_indefinite = static_cast<int32_t>(counter);
+    raise_status(status);
+    return value;
+}
+
+void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g, double* H,
+                  double* energies, hipStream_t stream) {
     if (kind < -1 || kind > 1)
-        lpg::fail(lpg::kInvalid, "Energy: invalid Hessian kind");
-    if (!value)
-        lpg::fail(lpg::kInvalid, "lpg_tet_energy_eval: value pointer is null");
+        fail(kInvalid, "Energy: invalid Hessian kind");
     if ((want_g && !g && h->setup.num_dofs > 0) || (kind >= 0 && !H && h->setup.nnz > 0) ||
         (!x && h->setup.num_dofs > 0))
-        lpg::fail(lpg::kInvalid, "lpg_tet_energy_eval: missing input or output buffer");
+        fail(kInvalid, "lpg_tet_energy_eval: missing input or output buffer");
     if (!theta && !h->setup.theta_valid && h->setup.num_tets > 0)
-        lpg::fail(lpg::kInvalid, "NeoHookean requires finite lambda >= 0 and mu > 0");
+        fail(kInvalid, "NeoHookean requires finite lambda >= 0 and mu > 0");
     ensure_device(h);
-    lpg::EvalArgs a;
+    EvalArgs a;
     a.x = x;
     a.theta = theta;
     a.g = want_g ? g : nullptr;
     a.H = kind >= 0 ? H : nullptr;
     a.want_g = want_g;
     a.kind = kind;
-    a.partial_E = h->plan.d_partial.as<double>();
+    a.vertex_E = h->d_vertex_E.as<double>();
     a.energy = h->d_energy();
     a.status = h->d_status();
     a.counter = h->d_counter();
-    const lpg::DeviceMesh m = h->mesh();
+    const DeviceMesh m = h->mesh();
     if (kind == 1) {
         const size_t T = static_cast<size_t>(h->setup.num_tets);
         h->d_proj_slot.ensure(std::max<size_t>(1, T) * sizeof(int32_t));
@@ -288,54 +225,93 @@ void eval_device(lpg_tet_energy* h, const double* x, const double* theta, int wa
     }
     if (h->timing)
         hip_check(hipEventRecord(h->ev[0], stream), "hipEventRecord");
-    lpg::launch_clear(a.status, a.counter, stream);
+    launch_clear(a.status, a.counter, stream);
     if (kind == 1) {
-        lpg::launch_project_prepass(m, a, stream);
-        lpg::launch_project_jacobi(m, a, stream);
+        launch_project_prepass(m, a, stream);
+        launch_project_jacobi(m, a, stream);
     }
     if (h->timing)
         hip_check(hipEventRecord(h->ev[1], stream), "hipEventRecord");
     const TilePlan& p = h->plan;
     if (p.oversized_ids.empty()) {
-        lpg::launch_assemble(m, a, p.lc, p.d_tile_vbeg.as<int32_t>(), nullptr, p.num_tiles(), p.max_segment, false,
-                             stream);
+        launch_assemble(m, a, p.lc, p.d_tile_vbeg.as<int32_t>(), nullptr, p.num_tiles(), p.max_segment, false,
+                        p.persistent_grid, stream);
     } else {
-        lpg::launch_assemble(m, a, p.lc, p.d_tile_vbeg.as<int32_t>(), p.d_normal_ids.as<int32_t>(),
-                             static_cast<int>(p.normal_ids.size()), p.max_segment, false, stream);
-        lpg::launch_assemble(m, a, p.lc, p.d_tile_vbeg.as<int32_t>(), p.d_oversized_ids.as<int32_t>(),
-                             static_cast<int>(p.oversized_ids.size()), 0, true, stream);
+        launch_assemble(m, a, p.lc, p.d_tile_vbeg.as<int32_t>(), p.d_normal_ids.as<int32_t>(),
+                        static_cast<int>(p.normal_ids.size()), p.max_segment, false, p.persistent_grid, stream);
+        launch_assemble(m, a, p.lc, p.d_tile_vbeg.as<int32_t>(), p.d_oversized_ids.as<int32_t>(),
+                        static_cast<int>(p.oversized_ids.size()), 0, true, 0, stream);
     }
     if (h->timing)
         hip_check(hipEventRecord(h->ev[2], stream), "hipEventRecord");
-    lpg::launch_energy_reduce(a.partial_E, p.num_tiles(), a.energy, a.status, stream);
+    launch_energy_reduce(a.vertex_E, h->d_chunk_begin.as<int32_t>(), static_cast<int>(h->chunk_begin.size()) - 1,
+                         h->d_chunk_sum.as<double>(), h->d_seg_chunk.as<int32_t>(),
+                         static_cast<int>(h->seg_chunk.size()) - 1, energies ? energies : a.energy, a.status,
+                         stream);
     if (h->timing)
         hip_check(hipEventRecord(h->ev[3], stream), "hipEventRecord");
+}
+
+void collect_timing(lpg_tet_energy* h) {
+    if (!h->timing)
+        return;
+    float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
+    hip_check(hipEventElapsedTime(&t01, h->ev[0], h->ev[1]), "hipEventElapsedTime");
+    hip_check(hipEventElapsedTime(&t12, h->ev[1], h->ev[2]), "hipEventElapsedTime");
+    hip_check(hipEventElapsedTime(&t23, h->ev[2], h->ev[3]), "hipEventElapsedTime");
+    hip_check(hipEventElapsedTime(&t03, h->ev[0], h->ev[3]), "hipEventElapsedTime");
+    h->time_ms[0] += t01;
+    h->time_ms[1] += t12;
+    h->time_ms[2] += t23;
+    h->time_ms[3] += t03;
+    ++h->timed_evals;
+}
+
+namespace {
+
+void eval_device(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* value,
+                 double* g, double* H, hipStream_t stream) {
+    if (!value)
+        fail(kInvalid, "lpg_tet_energy_eval: value pointer is null");
+    enqueue_eval(h, x, theta, want_g, kind, g, H, nullptr, stream);
     *value = finish(h, stream, kind == 1);
-    if (h->timing) {
-        float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
-        hip_check(hipEventElapsedTime(&t01, h->ev[0], h->ev[1]), "hipEventElapsedTime");
-        hip_check(hipEventElapsedTime(&t12, h->ev[1], h->ev[2]), "hipEventElapsedTime");
-        hip_check(hipEventElapsedTime(&t23, h->ev[2], h->ev[3]), "hipEventElapsedTime");
-        hip_check(hipEventElapsedTime(&t03, h->ev[0], h->ev[3]), "hipEventElapsedTime");
-        h->time_ms[0] += t01;
-        h->time_ms[1] += t12;
-        h->time_ms[2] += t23;
-        h->time_ms[3] += t03;
-        ++h->timed_evals;
-    }
+    collect_timing(h);
+}
+
+template <class Launch>
+lpg_status derivative_call(lpg_tet_energy* h, const double* x, const double* theta, void* stream, Launch&& launch) {
+    return guarded([&] {
+        if (!h)
+            fail(kInvalid, "null handle");
+        BusyScope scope(h->busy);
+        if (!x && h->setup.num_dofs > 0)
+            fail(kInvalid, "Energy::eval: x must have num_dofs finite entries");
+        if (!theta && !h->setup.theta_valid && h->setup.num_tets > 0)
+            fail(kInvalid, "NeoHookean requires finite lambda >= 0 and mu > 0");
+        ensure_device(h);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        launch_clear(h->d_status(), h->d_counter(), s);
+        launch(h->mesh(), h->d_status(), s);
+        (void)finish(h, s, false);
+    });
 }
 
 } // namespace
+} // namespace lpg
+
+using lpg::fail;
+using lpg::guarded;
 
 extern "C" {
 
 size_t lpg_last_error(char* buf, size_t len) {
+    const std::string& msg = lpg::g_last_error;
     if (buf && len > 0) {
-        const size_t n = std::min(len - 1, g_last_error.size());
-        std::memcpy(buf, g_last_error.data(), n);
+        const size_t n = std::min(len - 1, msg.size());
+        std::memcpy(buf, msg.data(), n);
         buf[n] = 0;
     }
-    return g_last_error.size();
+    return msg.size();
 }
 
 lpg_status lpg_tet_energy_create(int32_t num_vertices, int32_t num_tets, const int32_t* tets,
@@ -343,12 +319,12 @@ lpg_status lpg_tet_energy_create(int32_t num_vertices, int32_t num_tets, const i
                                  int32_t batch_size, lpg_tet_energy** out) {
     return guarded([&] {
         if (!out)
-            lpg::fail(lpg::kInvalid, "lpg_tet_energy_create: out is null");
+            fail(lpg::kInvalid, "lpg_tet_energy_create: out is null");
         *out = nullptr;
         if (batch_size <= 0)
-            lpg::fail(lpg::kInvalid, "MeshEnergy: invalid batch size");
+            fail(lpg::kInvalid, "MeshEnergy: invalid batch size");
         auto h = std::make_unique<lpg_tet_energy>();
-        const int nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        const int nthreads = static_cast<int>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
         lpg::build_mesh_setup(h->setup, num_vertices, num_tets, tets, rest_positions, theta, Fp, nthreads);
         *out = h.release();
     });
@@ -371,7 +347,7 @@ int32_t lpg_tet_energy_nnz(const lpg_tet_energy* h) {
 lpg_status lpg_tet_energy_pattern(const lpg_tet_energy* h, int32_t* row_ptr, int32_t* col_idx) {
     return guarded([&] {
         if (!h || !row_ptr || (!col_idx && h->setup.nnz > 0))
-            lpg::fail(lpg::kInvalid, "lpg_tet_energy_pattern: null argument");
+            fail(lpg::kInvalid, "lpg_tet_energy_pattern: null argument");
         std::copy(h->setup.row_ptr.begin(), h->setup.row_ptr.end(), row_ptr);
         std::copy(h->setup.col_idx.begin(), h->setup.col_idx.end(), col_idx);
     });
@@ -380,7 +356,7 @@ lpg_status lpg_tet_energy_pattern(const lpg_tet_energy* h, int32_t* row_ptr, int
 lpg_status lpg_tet_energy_slot_map(const lpg_tet_energy* h, int32_t* offsets) {
     return guarded([&] {
         if (!h || (!offsets && h->setup.num_tets > 0))
-            lpg::fail(lpg::kInvalid, "lpg_tet_energy_slot_map: null argument");
+            fail(lpg::kInvalid, "lpg_tet_energy_slot_map: null argument");
         lpg::build_slot_map(h->setup, offsets);
     });
 }
@@ -389,32 +365,31 @@ lpg_status lpg_tet_energy_eval(lpg_tet_energy* h, const double* x, int32_t want_
                                double* value, double* gradient, double* hessian_values) {
     return guarded([&] {
         if (!h)
-            lpg::fail(lpg::kInvalid, "lpg_tet_energy_eval: null handle");
-        BusyScope scope(h->busy);
+            fail(lpg::kInvalid, "lpg_tet_energy_eval: null handle");
+        lpg::BusyScope scope(h->busy);
         const auto& s = h->setup;
         if (!x && s.num_dofs > 0)
-            lpg::fail(lpg::kInvalid, "Energy::eval: x must have num_dofs finite entries");
+            fail(lpg::kInvalid, "Energy::eval: x must have num_dofs finite entries");
         for (int32_t i = 0; i < s.num_dofs; ++i)
             if (!std::isfinite(x[i]))
-                lpg::fail(lpg::kInvalid, "Energy::eval: x must have num_dofs finite entries");
-        if ((want_gradient && !gradient && s.num_dofs > 0) ||
-            (hessian_kind >= 0 && !hessian_values && s.nnz > 0))
-            lpg::fail(lpg::kInvalid, "lpg_tet_energy_eval: missing output buffer");
-        ensure_device(h);
-        upload(h->d_x, x, s.num_dofs * sizeof(double));
+                fail(lpg::kInvalid, "Energy::eval: x must have num_dofs finite entries");
+        if ((want_gradient && !gradient && s.num_dofs > 0) || (hessian_kind >= 0 && !hessian_values && s.nnz > 0))
+            fail(lpg::kInvalid, "lpg_tet_energy_eval: missing output buffer");
+        lpg::ensure_device(h);
+        lpg::upload(h->d_x, x, s.num_dofs * sizeof(double));
         if (want_gradient)
             h->d_g.ensure(s.num_dofs * sizeof(double));
         if (hessian_kind >= 0)
             h->d_H.ensure(static_cast<size_t>(s.nnz) * sizeof(double));
-        eval_device(h, h->d_x.as<double>(), nullptr, want_gradient, hessian_kind, value, h->d_g.as<double>(),
-                    h->d_H.as<double>(), h->own_stream);
+        lpg::eval_device(h, h->d_x.as<double>(), nullptr, want_gradient, hessian_kind, value, h->d_g.as<double>(),
+                         h->d_H.as<double>(), h->own_stream);
         if (want_gradient && s.num_dofs > 0)
-            hip_check(hipMemcpy(gradient, h->d_g.ptr, s.num_dofs * sizeof(double), hipMemcpyDeviceToHost),
-                      "gradient read-back");
+            lpg::hip_check(hipMemcpy(gradient, h->d_g.ptr, s.num_dofs * sizeof(double), hipMemcpyDeviceToHost),
+                           "gradient read-back");
         if (hessian_kind >= 0 && s.nnz > 0)
-            hip_check(hipMemcpy(hessian_values, h->d_H.ptr, static_cast<size_t>(s.nnz) * sizeof(double),
-                                hipMemcpyDeviceToHost),
-                      "Hessian read-back");
+            lpg::hip_check(hipMemcpy(hessian_values, h->d_H.ptr, static_cast<size_t>(s.nnz) * sizeof(double),
+                                     hipMemcpyDeviceToHost),
+                           "Hessian read-back");
     });
 }
 
@@ -423,10 +398,10 @@ lpg_status lpg_tet_energy_eval_device(lpg_tet_energy* h, const double* x_dev, co
                                       double* gradient_dev, double* hessian_values_dev, void* stream) {
     return guarded([&] {
         if (!h)
-            lpg::fail(lpg::kInvalid, "lpg_tet_energy_eval_device: null handle");
-        BusyScope scope(h->busy);
-        eval_device(h, x_dev, theta_dev, want_gradient, hessian_kind, value, gradient_dev, hessian_values_dev,
-                    stream_of(h, stream));
+            fail(lpg::kInvalid, "lpg_tet_energy_eval_device: null handle");
+        lpg::BusyScope scope(h->busy);
+        lpg::eval_device(h, x_dev, theta_dev, want_gradient, hessian_kind, value, gradient_dev, hessian_values_dev,
+                         static_cast<hipStream_t>(stream));
     });
 }
 
@@ -434,56 +409,36 @@ int32_t lpg_tet_energy_last_indefinite_count(const lpg_tet_energy* h) {
     return h ? h->last_indefinite : 0;
 }
 
-} // extern "C"
-
-namespace {
-template <class Launch> lpg_status derivative_call(lpg_tet_energy* h, const double* x, const double* theta,
-                                                   void* stream, Launch&& launch) {
-    return guarded([&] {
-        if (!h)
-            lpg::fail(lpg::kInvalid, "null handle");
-        BusyScope scope(h->busy);
-        if (!x && h->setup.num_dofs > 0)
-            lpg::fail(lpg::kInvalid, "Energy::eval: x must have num_dofs finite entries");
-        if (!theta && !h->setup.theta_valid && h->setup.num_tets > 0)
-            lpg::fail(lpg::kInvalid, "NeoHookean requires finite lambda >= 0 and mu > 0");
-        ensure_device(h);
-        const hipStream_t s = stream_of(h, stream);
-        lpg::launch_clear(h->d_status(), h->d_counter(), s);
-        launch(h->mesh(), h->d_status(), s);
-        (void)finish(h, s, false);
-    });
-}
-} // namespace
-
-extern "C" {
-
 lpg_status lpg_tet_energy_dE_dtheta_device(lpg_tet_energy* h, const double* x_dev, const double* theta_dev,
                                            double* out_dev, void* stream) {
-    return derivative_call(h, x_dev, theta_dev, stream, [&](const lpg::DeviceMesh& m, unsigned* st, hipStream_t s) {
-        lpg::launch_dE_dtheta(m, x_dev, theta_dev, out_dev, st, s);
-    });
+    return lpg::derivative_call(h, x_dev, theta_dev, stream,
+                                [&](const lpg::DeviceMesh& m, unsigned* st, hipStream_t s) {
+                                    lpg::launch_dE_dtheta(m, x_dev, theta_dev, out_dev, st, s);
+                                });
 }
 
 lpg_status lpg_tet_energy_force_vjp_theta_device(lpg_tet_energy* h, const double* x_dev, const double* theta_dev,
                                                  const double* w_dev, double* out_dev, void* stream) {
-    return derivative_call(h, x_dev, theta_dev, stream, [&](const lpg::DeviceMesh& m, unsigned* st, hipStream_t s) {
-        lpg::launch_force_vjp_theta(m, x_dev, theta_dev, w_dev, out_dev, st, s);
-    });
+    return lpg::derivative_call(h, x_dev, theta_dev, stream,
+                                [&](const lpg::DeviceMesh& m, unsigned* st, hipStream_t s) {
+                                    lpg::launch_force_vjp_theta(m, x_dev, theta_dev, w_dev, out_dev, st, s);
+                                });
 }
 
 lpg_status lpg_tet_energy_force_jvp_theta_device(lpg_tet_energy* h, const double* x_dev, const double* theta_dev,
                                                  const double* dtheta_dev, double* out_dev, void* stream) {
-    return derivative_call(h, x_dev, theta_dev, stream, [&](const lpg::DeviceMesh& m, unsigned* st, hipStream_t s) {
-        lpg::launch_force_jvp_theta(m, x_dev, theta_dev, dtheta_dev, out_dev, st, s);
-    });
+    return lpg::derivative_call(h, x_dev, theta_dev, stream,
+                                [&](const lpg::DeviceMesh& m, unsigned* st, hipStream_t s) {
+                                    lpg::launch_force_jvp_theta(m, x_dev, theta_dev, dtheta_dev, out_dev, st, s);
+                                });
 }
 
 lpg_status lpg_tet_energy_hvp_device(lpg_tet_energy* h, const double* x_dev, const double* theta_dev,
                                      const double* v_dev, double* out_dev, void* stream) {
-    return derivative_call(h, x_dev, theta_dev, stream, [&](const lpg::DeviceMesh& m, unsigned* st, hipStream_t s) {
-        lpg::launch_hvp(m, x_dev, theta_dev, v_dev, out_dev, st, s);
-    });
+    return lpg::derivative_call(h, x_dev, theta_dev, stream,
+                                [&](const lpg::DeviceMesh& m, unsigned* st, hipStream_t s) {
+                                    lpg::launch_hvp(m, x_dev, theta_dev, v_dev, out_dev, st, s);
+                                });
 }
 
 // ---- extensions (litepgo_fem_ext.h) ----
@@ -491,25 +446,25 @@ lpg_status lpg_tet_energy_set_launch(lpg_tet_energy* h, int32_t lanes_per_vertex
                                      int32_t vertices_per_tile, int32_t lds_budget_doubles) {
     return guarded([&] {
         if (!h)
-            lpg::fail(lpg::kInvalid, "null handle");
-        BusyScope scope(h->busy);
+            fail(lpg::kInvalid, "null handle");
+        lpg::BusyScope scope(h->busy);
         if (!(lanes_per_vertex == 1 || lanes_per_vertex == 2 || lanes_per_vertex == 4) ||
             !(waves == 1 || waves == 2 || waves == 4) || vertices_per_tile <= 0 ||
             vertices_per_tile > 64 * waves / lanes_per_vertex || lds_budget_doubles <= 0 ||
             lds_budget_doubles > 20000 - 64 * waves)
-            lpg::fail(lpg::kInvalid, "lpg_tet_energy_set_launch: unsupported configuration");
-        ensure_device(h);
+            fail(lpg::kInvalid, "lpg_tet_energy_set_launch: unsupported configuration");
+        lpg::ensure_device(h);
         lpg::LaunchConfig lc;
         lc.lanes_per_vertex = lanes_per_vertex;
         lc.waves = waves;
-        set_plan(h, lc, vertices_per_tile, lds_budget_doubles);
+        lpg::set_plan(h, lc, vertices_per_tile, lds_budget_doubles, 0);
     });
 }
 
 lpg_status lpg_tet_energy_get_launch(const lpg_tet_energy* h, int32_t* cfg4) {
     return guarded([&] {
         if (!h || !cfg4)
-            lpg::fail(lpg::kInvalid, "null argument");
+            fail(lpg::kInvalid, "null argument");
         cfg4[0] = h->plan.lc.lanes_per_vertex;
         cfg4[1] = h->plan.lc.waves;
         cfg4[2] = h->plan.vertices_per_tile;
@@ -520,12 +475,12 @@ lpg_status lpg_tet_energy_get_launch(const lpg_tet_energy* h, int32_t* cfg4) {
 lpg_status lpg_tet_energy_set_timing(lpg_tet_energy* h, int32_t enable) {
     return guarded([&] {
         if (!h)
-            lpg::fail(lpg::kInvalid, "null handle");
-        BusyScope scope(h->busy);
-        ensure_device(h);
+            fail(lpg::kInvalid, "null handle");
+        lpg::BusyScope scope(h->busy);
+        lpg::ensure_device(h);
         if (enable && !h->ev[0])
             for (hipEvent_t& e : h->ev)
-                hip_check(hipEventCreate(&e), "hipEventCreate");
+                lpg::hip_check(hipEventCreate(&e), "hipEventCreate");
         h->timing = enable != 0;
         for (double& t : h->time_ms)
             t = 0;
@@ -536,7 +491,7 @@ lpg_status lpg_tet_energy_set_timing(lpg_tet_energy* h, int32_t enable) {
 lpg_status lpg_tet_energy_kernel_times(const lpg_tet_energy* h, double* ms, int32_t* evals) {
     return guarded([&] {
         if (!h || !ms || !evals)
-            lpg::fail(lpg::kInvalid, "null argument");
+            fail(lpg::kInvalid, "null argument");
         for (int k = 0; k < 4; ++k)
             ms[k] = h->time_ms[k];
         *evals = h->timed_evals;

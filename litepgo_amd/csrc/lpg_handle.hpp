@@ -1,0 +1,125 @@
+// Host-side state of an energy handle and the helpers shared by the plain and batched C ABI entry points.
+#pragma once
+
+#include "litepgo_fem.h"
+#include "lpg_internal.hpp"
+
+#include <atomic>
+#include <string>
+#include <vector>
+
+namespace lpg {
+
+void hip_check(hipError_t err, const char* what);
+
+// Owning device allocation that only grows.
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t want);
+    void release();
+    template <class T> T* as() const {
+        return static_cast<T*>(ptr);
+    }
+};
+
+void upload(DeviceBuffer& buf, const void* src, size_t bytes);
+
+// Tiles of the assembly launch for one launch configuration.
+struct TilePlan {
+    LaunchConfig lc;
+    int vertices_per_tile = 0;
+    int lds_budget = 0;
+    int persistent_grid = 0; // 0: one workgroup per tile
+    std::vector<int32_t> tile_vbeg;
+    std::vector<int32_t> normal_ids, oversized_ids;
+    int32_t max_segment = 0;
+    DeviceBuffer d_tile_vbeg, d_normal_ids, d_oversized_ids;
+    int num_tiles() const {
+        return static_cast<int>(tile_vbeg.size()) - 1;
+    }
+};
+
+// Overlapping calls on one handle are rejected, as MeshEnergy's BusyScope does (mesh_energy.cpp:49).
+struct BusyScope {
+    std::atomic<bool>& flag;
+    explicit BusyScope(std::atomic<bool>& f);
+    ~BusyScope();
+};
+
+// Stores the message of a failed call for lpg_last_error and returns its status.
+template <class F> lpg_status guarded(F&& body);
+
+} // namespace lpg
+
+struct lpg_tet_energy {
+    lpg::MeshSetup setup;
+    // Vertex ranges of the meshes whose energies are reported separately: {0, V} for a plain energy, the mesh
+    // starts plus V for a batch.  Tiles never cross these boundaries.
+    std::vector<int32_t> seg_vbeg;
+    std::vector<int32_t> chunk_begin, seg_chunk; // energy reduction chunks (build_energy_chunks)
+    int device = 0;
+    bool device_ready = false;
+    hipStream_t own_stream = nullptr;
+    lpg::DeviceBuffer d_rec, d_inc_ptr, d_inc, d_row_ptr;
+    lpg::DeviceBuffer d_scalars; // energy (double) + status + counter
+    lpg::DeviceBuffer d_vertex_E, d_chunk_begin, d_chunk_sum, d_seg_chunk;
+    lpg::DeviceBuffer d_proj_slot, d_proj_list, d_proj_M;
+    lpg::DeviceBuffer d_x, d_g, d_H; // staging for the host-pointer API
+    lpg::TilePlan plan;
+    std::atomic<bool> busy{false};
+    int32_t last_indefinite = 0;
+    void* pinned = nullptr; // energy + status + counter read-back
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double time_ms[4] = {0, 0, 0, 0};
+    int32_t timed_evals = 0;
+
+    lpg::DeviceMesh mesh() const;
+    double* d_energy() const {
+        return d_scalars.as<double>();
+    }
+    unsigned* d_status() const {
+        return reinterpret_cast<unsigned*>(d_scalars.as<char>() + 8);
+    }
+    unsigned* d_counter() const {
+        return reinterpret_cast<unsigned*>(d_scalars.as<char>() + 12);
+    }
+    ~lpg_tet_energy();
+};
+
+namespace lpg {
+
+// Uploads the mesh and creates the stream and read-back buffer on first use.
+void ensure_device(lpg_tet_energy* h);
+// (Re)builds the tiles for a launch configuration.
+void set_plan(lpg_tet_energy* h, const LaunchConfig& lc, int vertices_per_tile, int lds_budget,
+              int persistent_grid);
+// Checks arguments and enqueues one evaluation (no synchronisation).  energies (device) receives one energy per
+// vertex segment (seg_vbeg); null means the handle's own scalar slot (plain energies).
+void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g, double* H,
+                  double* energies, hipStream_t stream);
+// Enqueues the read-back of status, counter and the first energy, synchronises, maps the status to an error.
+double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter);
+// The status mapping alone (status word as read back).
+void raise_status(unsigned status);
+
+extern thread_local std::string g_last_error;
+
+template <class F> lpg_status guarded(F&& body) {
+    try {
+        body();
+        return LPG_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return static_cast<lpg_status>(e.code);
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return LPG_DEVICE_ERROR;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return LPG_LOGIC_ERROR;
+    }
+}
+
+} // namespace lpg

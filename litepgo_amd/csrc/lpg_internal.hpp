@@ -82,8 +82,10 @@ struct TileConfig {
     int vertices_per_tile;    // upper bound on vertices in a tile
     int lds_budget_doubles;   // upper bound on the H segment staged in LDS
 };
+// Tiles never cross a vertex listed in `breaks` (mesh boundaries of a batch); null means no breaks.
 void build_tiles(const MeshSetup& s, const TileConfig& cfg, std::vector<int32_t>& tile_vbeg,
-                 int32_t& max_segment, std::vector<int32_t>* oversized);
+                 int32_t& max_segment, std::vector<int32_t>* oversized,
+                 const std::vector<int32_t>* breaks = nullptr);
 
 // ------------------------------------------------------------------------------------------------------------
 // Kernel parameters and launchers (lpg_kernels.hip).
@@ -106,7 +108,7 @@ struct EvalArgs {
     int want_g = 0;
     int kind = -1;
     // scratch
-    double* partial_E = nullptr; // one per tile
+    double* vertex_E = nullptr;  // one per vertex
     double* energy = nullptr;    // 1 double (device)
     unsigned* status = nullptr;  // 1 word (device)
     unsigned* counter = nullptr; // indefinite count (device)
@@ -123,9 +125,19 @@ struct LaunchConfig {
 void launch_clear(unsigned* status, unsigned* counter, hipStream_t stream);
 // Processes tiles tile_ids[0..num_tiles) (or 0..num_tiles-1 when tile_ids is null).  global_acc accumulates in
 // the H output itself instead of LDS (for tiles whose segment exceeds the LDS budget).
+// persistent_grid > 0 launches that many workgroups, each walking the tile list with a grid stride.
 void launch_assemble(const DeviceMesh& m, const EvalArgs& a, const LaunchConfig& lc, const int32_t* tile_vbeg,
-                     const int32_t* tile_ids, int num_tiles, int lds_doubles, bool global_acc, hipStream_t stream);
-void launch_energy_reduce(const double* partial, int n, double* out, unsigned* status, hipStream_t stream);
+                     const int32_t* tile_ids, int num_tiles, int lds_doubles, bool global_acc, int persistent_grid,
+                     hipStream_t stream);
+// Energy per mesh from the per-vertex energies: chunk c sums vertex_E[chunk_begin[c] .. chunk_begin[c+1]),
+// segment m sums chunk_sum[seg_chunk[m] .. seg_chunk[m+1]) into out[m].
+void launch_energy_reduce(const double* vertex_E, const int32_t* chunk_begin, int num_chunks, double* chunk_sum,
+                          const int32_t* seg_chunk, int num_segments, double* out, unsigned* status,
+                          hipStream_t stream);
+// Chunks of at most 1024 vertices, restarting at every segment start: chunk_begin (chunks + 1 entries) and
+// seg_chunk (segments + 1 entries) for segment starts seg_vbeg (segments + 1 entries, the last = num_vertices).
+void build_energy_chunks(const std::vector<int32_t>& seg_vbeg, std::vector<int32_t>& chunk_begin,
+                         std::vector<int32_t>& seg_chunk);
 void launch_project_prepass(const DeviceMesh& m, const EvalArgs& a, hipStream_t stream);
 void launch_project_jacobi(const DeviceMesh& m, const EvalArgs& a, hipStream_t stream);
 void launch_dE_dtheta(const DeviceMesh& m, const double* x, const double* theta, double* out, unsigned* status,

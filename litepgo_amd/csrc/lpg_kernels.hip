@@ -25,8 +25,17 @@
 
 #include <cmath>
 
+// No implicit FMA contraction: every fused multiply-add below is an explicit fma(), so each value is computed by the
+// same operation sequence in every template instantiation and launch shape, and results are bitwise reproducible
+// across configurations.
+#pragma clang fp contract(off)
+
 namespace lpg {
 namespace {
+
+__device__ __forceinline__ double dot3(double a0, double a1, double a2, double b0, double b1, double b2) {
+    return fma(a2, b2, fma(a1, b1, a0 * b0));
+}
 
 constexpr int kWave = 64;
 
@@ -113,9 +122,9 @@ __device__ __forceinline__ void kinematics(const double u[4][3], const double Di
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
-            C[3 * i + j] = k.F[3 * i1 + j1] * k.F[3 * i2 + j2] - k.F[3 * i1 + j2] * k.F[3 * i2 + j1];
+            C[3 * i + j] = fma(k.F[3 * i1 + j1], k.F[3 * i2 + j2], -(k.F[3 * i1 + j2] * k.F[3 * i2 + j1]));
         }
-    k.J = k.F[0] * C[0] + k.F[1] * C[1] + k.F[2] * C[2];
+    k.J = dot3(k.F[0], k.F[1], k.F[2], C[0], C[1], C[2]);
     const double inv = 1.0 / k.J;
 #pragma unroll
     for (int q = 0; q < 9; ++q)
@@ -190,19 +199,19 @@ struct AssembleParams {
     const double* theta;
     double* g;
     double* H;
-    double* partial_E;
+    double* vertex_E; // per-vertex element-order sum of the energies of the tets whose slot 0 it is
     unsigned* status;
     const int32_t* proj_slot;
     const double* proj_M;
 };
 
+// One tile: zero the tile's H segment in LDS (or in HBM for GLOBAL_ACC), accumulate, write it back coalesced,
+// write the gradient rows and the per-vertex energies, and OR the status flags.
 template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GLOBAL_ACC>
-__global__ __launch_bounds__(kWave* WAVES) void k_assemble(AssembleParams p) {
-    extern __shared__ double lds[];
+__device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile, double* lds) {
     using Inc = typename IncT<WIDE>::type;
     const Inc* __restrict__ inc = static_cast<const Inc*>(p.inc);
     constexpr int kThreads = kWave * WAVES;
-    const int tile = p.tile_ids ? p.tile_ids[blockIdx.x] : static_cast<int>(blockIdx.x);
     const int vb = p.tile_vbeg[tile], ve = p.tile_vbeg[tile + 1];
     const int seg0 = KIND >= 0 ? p.row_ptr[3 * vb] : 0;
     const int seg1 = KIND >= 0 ? p.row_ptr[3 * ve] : 0;
@@ -243,30 +252,30 @@ __global__ __launch_bounds__(kWave* WAVES) void k_assemble(AssembleParams p) {
             shape_gradients(t.Di, G);
             double ga[3];
             select_row(G, a, ga);
-            const double p_ = t.lam * kin.L - t.mu;
+            const double p_ = fma(t.lam, kin.L, -t.mu);
             // q_a = F^-T g_a
             double qa[3];
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                qa[i] = kin.a[3 * i] * ga[0] + kin.a[3 * i + 1] * ga[1] + kin.a[3 * i + 2] * ga[2];
+                qa[i] = dot3(kin.a[3 * i], kin.a[3 * i + 1], kin.a[3 * i + 2], ga[0], ga[1], ga[2]);
             if (sub == 0) {
                 if (WANT_G) {
                     // w (mu F g_a + p q_a)
                     double Fg[3];
 #pragma unroll
                     for (int i = 0; i < 3; ++i)
-                        Fg[i] = kin.F[3 * i] * ga[0] + kin.F[3 * i + 1] * ga[1] + kin.F[3 * i + 2] * ga[2];
-                    g0 += t.w * (t.mu * Fg[0] + p_ * qa[0]);
-                    g1 += t.w * (t.mu * Fg[1] + p_ * qa[1]);
-                    g2 += t.w * (t.mu * Fg[2] + p_ * qa[2]);
+                        Fg[i] = dot3(kin.F[3 * i], kin.F[3 * i + 1], kin.F[3 * i + 2], ga[0], ga[1], ga[2]);
+                    g0 = fma(t.w, fma(t.mu, Fg[0], p_ * qa[0]), g0);
+                    g1 = fma(t.w, fma(t.mu, Fg[1], p_ * qa[1]), g1);
+                    g2 = fma(t.w, fma(t.mu, Fg[2], p_ * qa[2]), g2);
                 }
                 if (a == 0) {
                     double sq = 0.0;
 #pragma unroll
                     for (int q = 0; q < 9; ++q)
                         sq = fma(kin.F[q], kin.F[q], sq);
-                    const double psi = 0.5 * t.mu * (sq - 3.0) - t.mu * kin.L + 0.5 * t.lam * kin.L * kin.L;
-                    esum += t.w * psi;
+                    const double psi = fma(0.5 * t.lam * kin.L, kin.L, fma(0.5 * t.mu, sq - 3.0, -(t.mu * kin.L)));
+                    esum = fma(t.w, psi, esum);
                 }
             }
             if (KIND >= 0) {
@@ -292,14 +301,14 @@ __global__ __launch_bounds__(kWave* WAVES) void k_assemble(AssembleParams p) {
                         double qb[3];
 #pragma unroll
                         for (int i = 0; i < 3; ++i)
-                            qb[i] = kin.a[3 * i] * G[b][0] + kin.a[3 * i + 1] * G[b][1] + kin.a[3 * i + 2] * G[b][2];
-                        const double s1 = sw * (ga[0] * G[b][0] + ga[1] * G[b][1] + ga[2] * G[b][2]);
+                            qb[i] = dot3(kin.a[3 * i], kin.a[3 * i + 1], kin.a[3 * i + 2], G[b][0], G[b][1], G[b][2]);
+                        const double s1 = sw * dot3(ga[0], ga[1], ga[2], G[b][0], G[b][1], G[b][2]);
                         double* blk = row + 3 * cp;
 #pragma unroll
                         for (int i = 0; i < 3; ++i)
 #pragma unroll
                             for (int kk = 0; kk < 3; ++kk) {
-                                double val = xa[i] * qb[kk] - qb[i] * ya[kk];
+                                double val = fma(xa[i], qb[kk], -(qb[i] * ya[kk]));
                                 if (i == kk)
                                     val += s1;
                                 accum<GLOBAL_ACC>(blk + i * rowlen + kk, val);
@@ -346,8 +355,8 @@ __global__ __launch_bounds__(kWave* WAVES) void k_assemble(AssembleParams p) {
                         for (int i = 0; i < 3; ++i)
 #pragma unroll
                             for (int kk = 0; kk < 3; ++kk) {
-                                const double val = kHelmert[b][0] * R[0][i][kk] + kHelmert[b][1] * R[1][i][kk] +
-                                                   kHelmert[b][2] * R[2][i][kk];
+                                const double val = dot3(kHelmert[b][0], kHelmert[b][1], kHelmert[b][2],
+                                                        R[0][i][kk], R[1][i][kk], R[2][i][kk]);
                                 accum<GLOBAL_ACC>(blk + i * rowlen + kk, val);
                             }
                         }
@@ -355,6 +364,8 @@ __global__ __launch_bounds__(kWave* WAVES) void k_assemble(AssembleParams p) {
                 }
             }
         }
+        if (sub == 0)
+            p.vertex_E[v] = esum;
         if (WANT_G && sub == 0) {
             double* gv = p.g + 3 * static_cast<size_t>(v);
             gv[0] = g0;
@@ -379,20 +390,26 @@ __global__ __launch_bounds__(kWave* WAVES) void k_assemble(AssembleParams p) {
                     flags |= kStatusNonFiniteOut;
         }
     }
-    // Energy: fixed-order tree reduction of the per-lane sums.
-    __shared__ double red[kThreads];
-    red[threadIdx.x] = esum;
-    __syncthreads();
-#pragma unroll
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-        if (static_cast<int>(threadIdx.x) < s)
-            red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        p.partial_E[tile] = red[0];
     if (flags)
         atomicOr(p.status, flags);
+}
+
+template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GLOBAL_ACC>
+__global__ __launch_bounds__(kWave* WAVES) void k_assemble(AssembleParams p) {
+    extern __shared__ double lds[];
+    const int tile = p.tile_ids ? p.tile_ids[blockIdx.x] : static_cast<int>(blockIdx.x);
+    assemble_tile<LPV, WAVES, WANT_G, KIND, WIDE, GLOBAL_ACC>(p, tile, lds);
+}
+
+// Persistent form: a fixed grid walks the tile list with a grid stride (the batched launch).
+template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GLOBAL_ACC>
+__global__ __launch_bounds__(kWave* WAVES) void k_assemble_persistent(AssembleParams p, int num_tiles) {
+    extern __shared__ double lds[];
+    for (int t = blockIdx.x; t < num_tiles; t += gridDim.x) {
+        const int tile = p.tile_ids ? p.tile_ids[t] : t;
+        assemble_tile<LPV, WAVES, WANT_G, KIND, WIDE, GLOBAL_ACC>(p, tile, lds);
+        __syncthreads();
+    }
 }
 
 __global__ void k_clear(unsigned* status, unsigned* counter) {
@@ -403,23 +420,48 @@ __global__ void k_clear(unsigned* status, unsigned* counter) {
     }
 }
 
-// Deterministic sum of n partials: fixed strided order per lane, then a fixed tree.
-__global__ __launch_bounds__(256) void k_energy_reduce(const double* __restrict__ partial, int n,
-                                                       double* __restrict__ out, unsigned* status) {
-    __shared__ double red[256];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256)
-        s += partial[i];
+// Energy reduction, independent of the tiling: level 1 sums fixed chunks of at most 1024 per-vertex values
+// (chunk boundaries are relative to the first vertex of each mesh), level 2 sums each mesh's chunk partials.  Both
+// levels use a fixed strided order per lane and a fixed tree, so the energy of a mesh has the same bits whatever the
+// launch shape, and whether the mesh is evaluated alone or inside a batch.
+template <int N> __device__ __forceinline__ double tree_sum(double* red, double s) {
     red[threadIdx.x] = s;
     __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
+#pragma unroll
+    for (int k = N / 2; k > 0; k >>= 1) {
         if (static_cast<int>(threadIdx.x) < k)
             red[threadIdx.x] += red[threadIdx.x + k];
         __syncthreads();
     }
+    return red[0];
+}
+
+__global__ __launch_bounds__(256) void k_energy_chunks(const double* __restrict__ values,
+                                                       const int32_t* __restrict__ chunk_begin,
+                                                       double* __restrict__ chunk_sum) {
+    __shared__ double red[256];
+    const int c = blockIdx.x;
+    const int b = chunk_begin[c], e = chunk_begin[c + 1];
+    double s = 0.0;
+    for (int i = b + static_cast<int>(threadIdx.x); i < e; i += 256)
+        s += values[i];
+    const double total = tree_sum<256>(red, s);
+    if (threadIdx.x == 0)
+        chunk_sum[c] = total;
+}
+
+__global__ __launch_bounds__(256) void k_energy_segments(const double* __restrict__ chunk_sum,
+                                                         const int32_t* __restrict__ seg_chunk,
+                                                         double* __restrict__ out, unsigned* status) {
+    __shared__ double red[256];
+    const int m = blockIdx.x;
+    double s = 0.0;
+    for (int i = seg_chunk[m] + static_cast<int>(threadIdx.x); i < seg_chunk[m + 1]; i += 256)
+        s += chunk_sum[i];
+    const double total = tree_sum<256>(red, s);
     if (threadIdx.x == 0) {
-        out[0] = red[0];
-        if (!isfinite(red[0]))
+        out[m] = total;
+        if (!isfinite(total))
             atomicOr(status, kStatusNonFiniteOut);
     }
 }
@@ -456,7 +498,7 @@ __global__ __launch_bounds__(256) void k_project_prepass(ProjectParams p) {
     kinematics(u, t.Di, k);
     int32_t slot = -1;
     if (kin_ok(k) && theta_ok(t.lam, t.mu)) {
-        const double pr = t.lam * k.L - t.mu;
+        const double pr = fma(t.lam, k.L, -t.mu);
         // A(i+3j, m+3l) = mu d_im d_jl + lambda a_ij a_ml - p a_il a_mj, packed upper (45).
         double A[45];
 #pragma unroll
@@ -464,7 +506,7 @@ __global__ __launch_bounds__(256) void k_project_prepass(ProjectParams p) {
 #pragma unroll
             for (int c = r; c < 9; ++c) {
                 const int i = r % 3, j = r / 3, m = c % 3, l = c / 3;
-                double val = t.lam * k.a[3 * i + j] * k.a[3 * m + l] - pr * k.a[3 * i + l] * k.a[3 * m + j];
+                double val = fma(t.lam * k.a[3 * i + j], k.a[3 * m + l], -(pr * k.a[3 * i + l] * k.a[3 * m + j]));
                 if (i == m && j == l)
                     val += t.mu;
                 A[sym_index(r, c)] = val;
@@ -509,20 +551,20 @@ __global__ __launch_bounds__(64) void k_project_jacobi(ProjectParams p) {
         kinematics(u, t.Di, k);
         double G[4][3];
         shape_gradients(t.Di, G);
-        const double pr = t.lam * k.L - t.mu;
+        const double pr = fma(t.lam, k.L, -t.mu);
         // c_m = sum_a P(a,m) g_a, r_m = F^-T c_m.
         double cm[3][3], rm[3][3];
 #pragma unroll
         for (int m = 0; m < 3; ++m)
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-                cm[m][j] = kHelmert[0][m] * G[0][j] + kHelmert[1][m] * G[1][j] + kHelmert[2][m] * G[2][j] +
-                           kHelmert[3][m] * G[3][j];
+                cm[m][j] = fma(kHelmert[3][m], G[3][j],
+                               fma(kHelmert[2][m], G[2][j], fma(kHelmert[1][m], G[1][j], kHelmert[0][m] * G[0][j])));
 #pragma unroll
         for (int m = 0; m < 3; ++m)
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                rm[m][i] = k.a[3 * i] * cm[m][0] + k.a[3 * i + 1] * cm[m][1] + k.a[3 * i + 2] * cm[m][2];
+                rm[m][i] = dot3(k.a[3 * i], k.a[3 * i + 1], k.a[3 * i + 2], cm[m][0], cm[m][1], cm[m][2]);
         // M[(3m+i), (3n+k)] = w [mu (c_m . c_n) d_ik + lambda r_m,i r_n,k - p r_n,i r_m,k], packed upper.
         double M[45];
 #pragma unroll
@@ -530,9 +572,9 @@ __global__ __launch_bounds__(64) void k_project_jacobi(ProjectParams p) {
 #pragma unroll
             for (int c = r; c < 9; ++c) {
                 const int m = r / 3, i = r % 3, n = c / 3, kk = c % 3;
-                double val = t.lam * rm[m][i] * rm[n][kk] - pr * rm[n][i] * rm[m][kk];
+                double val = fma(t.lam * rm[m][i], rm[n][kk], -(pr * rm[n][i] * rm[m][kk]));
                 if (i == kk)
-                    val += t.mu * (cm[m][0] * cm[n][0] + cm[m][1] * cm[n][1] + cm[m][2] * cm[n][2]);
+                    val = fma(t.mu, dot3(cm[m][0], cm[m][1], cm[m][2], cm[n][0], cm[n][1], cm[n][2]), val);
                 M[sym_index(r, c)] = t.w * val;
             }
         double V[9][9];
@@ -572,17 +614,17 @@ __global__ __launch_bounds__(64) void k_project_jacobi(ProjectParams p) {
                             if (r == pp || r == qq)
                                 continue;
                             const double mrp = M[sym_index(r, pp)], mrq = M[sym_index(r, qq)];
-                            M[sym_index(r, pp)] = c * mrp - s * mrq;
-                            M[sym_index(r, qq)] = s * mrp + c * mrq;
+                            M[sym_index(r, pp)] = fma(c, mrp, -(s * mrq));
+                            M[sym_index(r, qq)] = fma(s, mrp, c * mrq);
                         }
-                        M[sym_index(pp, pp)] = app - tt * apq;
-                        M[sym_index(qq, qq)] = aqq + tt * apq;
+                        M[sym_index(pp, pp)] = fma(-tt, apq, app);
+                        M[sym_index(qq, qq)] = fma(tt, apq, aqq);
                         M[sym_index(pp, qq)] = 0.0;
 #pragma unroll
                         for (int r = 0; r < 9; ++r) {
                             const double vrp = V[r][pp], vrq = V[r][qq];
-                            V[r][pp] = c * vrp - s * vrq;
-                            V[r][qq] = s * vrp + c * vrq;
+                            V[r][pp] = fma(c, vrp, -(s * vrq));
+                            V[r][qq] = fma(s, vrp, c * vrq);
                         }
                     }
                 }
@@ -688,11 +730,11 @@ __global__ __launch_bounds__(256) void k_force_vjp_theta(ElemParams p) {
         double q[3], Fg[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            q[i] = k.a[3 * i] * G[b][0] + k.a[3 * i + 1] * G[b][1] + k.a[3 * i + 2] * G[b][2];
-            Fg[i] = k.F[3 * i] * G[b][0] + k.F[3 * i + 1] * G[b][1] + k.F[3 * i + 2] * G[b][2];
+            q[i] = dot3(k.a[3 * i], k.a[3 * i + 1], k.a[3 * i + 2], G[b][0], G[b][1], G[b][2]);
+            Fg[i] = dot3(k.F[3 * i], k.F[3 * i + 1], k.F[3 * i + 2], G[b][0], G[b][1], G[b][2]);
         }
-        const double qw = q[0] * w0 + q[1] * w1 + q[2] * w2;
-        const double fw = Fg[0] * w0 + Fg[1] * w1 + Fg[2] * w2;
+        const double qw = dot3(q[0], q[1], q[2], w0, w1, w2);
+        const double fw = dot3(Fg[0], Fg[1], Fg[2], w0, w1, w2);
         sl += qw;
         sm += fw - qw;
     }
@@ -748,9 +790,9 @@ template <int MODE> __global__ __launch_bounds__(256) void k_vertex_gather(Gathe
         double qa[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-            qa[i] = k.a[3 * i] * ga[0] + k.a[3 * i + 1] * ga[1] + k.a[3 * i + 2] * ga[2];
+            qa[i] = dot3(k.a[3 * i], k.a[3 * i + 1], k.a[3 * i + 2], ga[0], ga[1], ga[2]);
         if (MODE == 0) {
-            const double pr = t.lam * k.L - t.mu;
+            const double pr = fma(t.lam, k.L, -t.mu);
             // sum_b H_ab v_b = w sum_b [mu (g_a.g_b) v_b + lambda q_a (q_b.v_b) - p q_b (q_a.v_b)]
             double sv[3] = {0.0, 0.0, 0.0}, sq[3] = {0.0, 0.0, 0.0};
             double sqv = 0.0;
@@ -761,29 +803,30 @@ template <int MODE> __global__ __launch_bounds__(256) void k_vertex_gather(Gathe
                 double qb[3];
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
-                    qb[i] = k.a[3 * i] * G[b][0] + k.a[3 * i + 1] * G[b][1] + k.a[3 * i + 2] * G[b][2];
-                const double gab = ga[0] * G[b][0] + ga[1] * G[b][1] + ga[2] * G[b][2];
-                sv[0] += gab * vb0;
-                sv[1] += gab * vb1;
-                sv[2] += gab * vb2;
-                sqv += qb[0] * vb0 + qb[1] * vb1 + qb[2] * vb2;
-                const double qav = qa[0] * vb0 + qa[1] * vb1 + qa[2] * vb2;
-                sq[0] += qb[0] * qav;
-                sq[1] += qb[1] * qav;
-                sq[2] += qb[2] * qav;
+                    qb[i] = dot3(k.a[3 * i], k.a[3 * i + 1], k.a[3 * i + 2], G[b][0], G[b][1], G[b][2]);
+                const double gab = dot3(ga[0], ga[1], ga[2], G[b][0], G[b][1], G[b][2]);
+                sv[0] = fma(gab, vb0, sv[0]);
+                sv[1] = fma(gab, vb1, sv[1]);
+                sv[2] = fma(gab, vb2, sv[2]);
+                sqv += dot3(qb[0], qb[1], qb[2], vb0, vb1, vb2);
+                const double qav = dot3(qa[0], qa[1], qa[2], vb0, vb1, vb2);
+                sq[0] = fma(qb[0], qav, sq[0]);
+                sq[1] = fma(qb[1], qav, sq[1]);
+                sq[2] = fma(qb[2], qav, sq[2]);
             }
-            y0 += t.w * (t.mu * sv[0] + t.lam * qa[0] * sqv - pr * sq[0]);
-            y1 += t.w * (t.mu * sv[1] + t.lam * qa[1] * sqv - pr * sq[1]);
-            y2 += t.w * (t.mu * sv[2] + t.lam * qa[2] * sqv - pr * sq[2]);
+            y0 = fma(t.w, fma(t.mu, sv[0], fma(t.lam * qa[0], sqv, -(pr * sq[0]))), y0);
+            y1 = fma(t.w, fma(t.mu, sv[1], fma(t.lam * qa[1], sqv, -(pr * sq[1]))), y1);
+            y2 = fma(t.w, fma(t.mu, sv[2], fma(t.lam * qa[2], sqv, -(pr * sq[2]))), y2);
         } else {
             const double2 dt = reinterpret_cast<const double2*>(p.v)[e];
             double Fg[3];
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                Fg[i] = k.F[3 * i] * ga[0] + k.F[3 * i + 1] * ga[1] + k.F[3 * i + 2] * ga[2];
-            y0 += t.w * (dt.x * k.L * qa[0] + dt.y * (Fg[0] - qa[0]));
-            y1 += t.w * (dt.x * k.L * qa[1] + dt.y * (Fg[1] - qa[1]));
-            y2 += t.w * (dt.x * k.L * qa[2] + dt.y * (Fg[2] - qa[2]));
+                Fg[i] = dot3(k.F[3 * i], k.F[3 * i + 1], k.F[3 * i + 2], ga[0], ga[1], ga[2]);
+            const double dl = dt.x * k.L;
+            y0 = fma(t.w, fma(dl, qa[0], dt.y * (Fg[0] - qa[0])), y0);
+            y1 = fma(t.w, fma(dl, qa[1], dt.y * (Fg[1] - qa[1])), y1);
+            y2 = fma(t.w, fma(dl, qa[2], dt.y * (Fg[2] - qa[2])), y2);
         }
     }
     double* o = p.out + 3 * static_cast<size_t>(v);
@@ -802,35 +845,56 @@ void check_launch(const char* what) {
         fail(kDevice, std::string(what) + ": " + hipGetErrorString(err));
 }
 
+template <class Kernel> void allow_large_lds(Kernel kern, int waves) {
+    const hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               160 * 1024 - kWave * waves * 8);
+    if (err != hipSuccess)
+        fail(kDevice, std::string("hipFuncSetAttribute: ") + hipGetErrorString(err));
+}
+
 template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE>
 void launch_assemble_t(const AssembleParams& p, int num_tiles, int lds_doubles, bool global_acc,
-                       hipStream_t stream) {
-    const dim3 grid(num_tiles), block(kWave * WAVES);
-    if (global_acc) {
-        hipLaunchKernelGGL((k_assemble<LPV, WAVES, WANT_G, KIND, WIDE, true>), grid, block, 0, stream, p);
-    } else {
-        const size_t bytes = static_cast<size_t>(lds_doubles) * sizeof(double);
-        auto kern = k_assemble<LPV, WAVES, WANT_G, KIND, WIDE, false>;
-        static bool configured = false;
-        if (!configured) {
-            const hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                       160 * 1024 - kWave * WAVES * 8);
-            if (err != hipSuccess)
-                fail(kDevice, std::string("hipFuncSetAttribute: ") + hipGetErrorString(err));
-            configured = true;
+                       int persistent_grid, hipStream_t stream) {
+    const dim3 block(kWave * WAVES);
+    const size_t bytes = global_acc ? 0 : static_cast<size_t>(lds_doubles) * sizeof(double);
+    if (persistent_grid > 0) {
+        const dim3 grid(std::min(persistent_grid, num_tiles));
+        if (global_acc) {
+            hipLaunchKernelGGL((k_assemble_persistent<LPV, WAVES, WANT_G, KIND, WIDE, true>), grid, block, 0,
+                               stream, p, num_tiles);
+        } else {
+            auto kern = k_assemble_persistent<LPV, WAVES, WANT_G, KIND, WIDE, false>;
+            static bool configured = false;
+            if (!configured) {
+                allow_large_lds(kern, WAVES);
+                configured = true;
+            }
+            hipLaunchKernelGGL(kern, grid, block, bytes, stream, p, num_tiles);
         }
-        hipLaunchKernelGGL(kern, grid, block, bytes, stream, p);
+    } else {
+        const dim3 grid(num_tiles);
+        if (global_acc) {
+            hipLaunchKernelGGL((k_assemble<LPV, WAVES, WANT_G, KIND, WIDE, true>), grid, block, 0, stream, p);
+        } else {
+            auto kern = k_assemble<LPV, WAVES, WANT_G, KIND, WIDE, false>;
+            static bool configured = false;
+            if (!configured) {
+                allow_large_lds(kern, WAVES);
+                configured = true;
+            }
+            hipLaunchKernelGGL(kern, grid, block, bytes, stream, p);
+        }
     }
     check_launch("k_assemble");
 }
 
 template <int LPV, int WAVES>
 void dispatch_kind(const AssembleParams& p, int want_g, int kind, bool wide, int num_tiles, int lds, bool gacc,
-                   hipStream_t s) {
+                   int persistent_grid, hipStream_t s) {
 #define LPG_CASE(G, K, W)                                                                                            \
     if (!!want_g == G && kind == K && wide == W)                                                                      \
-        return launch_assemble_t<LPV, WAVES, G, K, W>(p, num_tiles, lds, gacc, s);
+        return launch_assemble_t<LPV, WAVES, G, K, W>(p, num_tiles, lds, gacc, persistent_grid, s);
     LPG_CASE(false, -1, false)
     LPG_CASE(true, -1, false)
     LPG_CASE(false, 0, false)
@@ -855,7 +919,8 @@ void launch_clear(unsigned* status, unsigned* counter, hipStream_t stream) {
 }
 
 void launch_assemble(const DeviceMesh& m, const EvalArgs& a, const LaunchConfig& lc, const int32_t* tile_vbeg,
-                     const int32_t* tile_ids, int num_tiles, int lds_doubles, bool global_acc, hipStream_t stream) {
+                     const int32_t* tile_ids, int num_tiles, int lds_doubles, bool global_acc, int persistent_grid,
+                     hipStream_t stream) {
     if (num_tiles <= 0)
         return;
     AssembleParams p{};
@@ -869,13 +934,14 @@ void launch_assemble(const DeviceMesh& m, const EvalArgs& a, const LaunchConfig&
     p.theta = a.theta;
     p.g = a.g;
     p.H = a.H;
-    p.partial_E = a.partial_E;
+    p.vertex_E = a.vertex_E;
     p.status = a.status;
     p.proj_slot = a.proj_slot;
     p.proj_M = a.proj_M;
 #define LPG_CFG(L, W)                                                                                                \
     if (lc.lanes_per_vertex == L && lc.waves == W)                                                                    \
-        return dispatch_kind<L, W>(p, a.want_g, a.kind, m.wide, num_tiles, lds_doubles, global_acc, stream);
+        return dispatch_kind<L, W>(p, a.want_g, a.kind, m.wide, num_tiles, lds_doubles, global_acc,           \
+                                   persistent_grid, stream);
     LPG_CFG(1, 1)
     LPG_CFG(2, 1)
     LPG_CFG(4, 1)
@@ -889,9 +955,18 @@ void launch_assemble(const DeviceMesh& m, const EvalArgs& a, const LaunchConfig&
     fail(kInvalid, "launch_assemble: unsupported launch configuration");
 }
 
-void launch_energy_reduce(const double* partial, int n, double* out, unsigned* status, hipStream_t stream) {
-    hipLaunchKernelGGL(k_energy_reduce, dim3(1), dim3(256), 0, stream, partial, n, out, status);
-    check_launch("k_energy_reduce");
+void launch_energy_reduce(const double* vertex_E, const int32_t* chunk_begin, int num_chunks, double* chunk_sum,
+                          const int32_t* seg_chunk, int num_segments, double* out, unsigned* status,
+                          hipStream_t stream) {
+    if (num_chunks > 0) {
+        hipLaunchKernelGGL(k_energy_chunks, dim3(num_chunks), dim3(256), 0, stream, vertex_E, chunk_begin, chunk_sum);
+        check_launch("k_energy_chunks");
+    }
+    if (num_segments > 0) {
+        hipLaunchKernelGGL(k_energy_segments, dim3(num_segments), dim3(256), 0, stream, chunk_sum, seg_chunk, out,
+                           status);
+        check_launch("k_energy_segments");
+    }
 }
 
 void launch_project_prepass(const DeviceMesh& m, const EvalArgs& a, hipStream_t stream) {

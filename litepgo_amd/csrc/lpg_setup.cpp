@@ -242,18 +242,22 @@ void build_slot_map(const MeshSetup& s, int32_t* offsets) {
 }
 
 void build_tiles(const MeshSetup& s, const TileConfig& cfg, std::vector<int32_t>& tile_vbeg, int32_t& max_segment,
-                 std::vector<int32_t>* oversized) {
+                 std::vector<int32_t>* oversized, const std::vector<int32_t>* breaks) {
     tile_vbeg.clear();
     max_segment = 0;
     if (oversized)
         oversized->clear();
     const int32_t nv = s.num_vertices;
     int32_t v = 0;
+    size_t next_break = 0;
     tile_vbeg.push_back(0);
     while (v < nv) {
         const int32_t start = v;
         int64_t seg = 0;
-        while (v < nv && v - start < cfg.vertices_per_tile) {
+        while (breaks && next_break < breaks->size() && (*breaks)[next_break] <= start)
+            ++next_break;
+        const int32_t limit = (breaks && next_break < breaks->size()) ? std::min(nv, (*breaks)[next_break]) : nv;
+        while (v < limit && v - start < cfg.vertices_per_tile) {
             const int64_t len = 9 * static_cast<int64_t>(s.nbr_ptr[v + 1] - s.nbr_ptr[v]);
             if (v > start && seg + len > cfg.lds_budget_doubles)
                 break;
@@ -265,6 +269,18 @@ void build_tiles(const MeshSetup& s, const TileConfig& cfg, std::vector<int32_t>
         else
             max_segment = std::max<int32_t>(max_segment, static_cast<int32_t>(seg));
         tile_vbeg.push_back(v);
+    }
+}
+
+void build_energy_chunks(const std::vector<int32_t>& seg_vbeg, std::vector<int32_t>& chunk_begin,
+                         std::vector<int32_t>& seg_chunk) {
+    constexpr int32_t kChunk = 1024;
+    chunk_begin.assign(1, seg_vbeg.empty() ? 0 : seg_vbeg.front());
+    seg_chunk.assign(1, 0);
+    for (size_t m = 0; m + 1 < seg_vbeg.size(); ++m) {
+        for (int32_t v = seg_vbeg[m]; v < seg_vbeg[m + 1]; v += kChunk)
+            chunk_begin.push_back(std::min(seg_vbeg[m + 1], v + kChunk));
+        seg_chunk.push_back(static_cast<int32_t>(chunk_begin.size()) - 1);
     }
 }
 

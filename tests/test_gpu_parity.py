@@ -151,30 +151,44 @@ def test_irregular_numbering_and_unused_vertex(oracle_mod):
     assert np.all(g[-3:] == 0.0)
 
 
-def test_high_degree_vertex_uses_wide_incidences(oracle_mod):
-    """A fan of 120 tets around one vertex: its degree (361) exceeds the 8-bit column-position form."""
-    rng = np.random.default_rng(4)
-    n = 120
+def fan_mesh(n, seed):
+    """n well-shaped tets sharing vertex 0 (distinct other vertices): vertex 0 has degree 3 n + 1."""
+    rng = np.random.default_rng(seed)
     X = [np.zeros(3)]
     tets = []
-    for e in range(n):
+    for _ in range(n):
         base = rng.normal(size=3)
         base /= np.linalg.norm(base)
-        pts = [base * 2 + rng.normal(scale=0.3, size=3) for _ in range(3)]
-        idx = list(range(len(X), len(X) + 3))
+        t1 = np.cross(base, rng.normal(size=3))
+        t1 /= np.linalg.norm(t1)
+        t2 = np.cross(base, t1)
+        pts = [2 * base + 0.6 * t1, 2 * base + 0.6 * t2, 2.6 * base]
+        t = [0, len(X), len(X) + 1, len(X) + 2]
         X.extend(pts)
-        t = [0] + idx
         D = np.stack([X[t[k]] - X[t[0]] for k in (1, 2, 3)], 1)
         if np.linalg.det(D) < 0:
             t[1], t[2] = t[2], t[1]
         tets.append(t)
-    X = np.array(X)
-    tets = np.array(tets, dtype=np.int32)
+    return np.array(X), np.array(tets, dtype=np.int32)
+
+
+def test_high_degree_vertex_uses_wide_incidences(oracle_mod):
+    """A fan of 120 tets around one vertex: its degree (361) exceeds the 8-bit column-position form."""
+    n = 120
+    X, tets = fan_mesh(n, 4)
     theta = np.tile([[1e4, 2e4]], (n, 1))
     dev, ref = make(oracle_mod, X, tets, theta)
     assert dev.max_degree == 3 * n + 1
-    x = rng.uniform(-0.01, 0.01, X.size)
+    x = np.random.default_rng(5).uniform(-0.01, 0.01, X.size)
     check_against_oracle(dev, ref, x)
+    # flattening vertex 0's tets inverts them: both sides raise the domain error
+    x = np.zeros(X.size)
+    x[3:] = -1.2 * X[1:].reshape(-1)
+    with pytest.raises(oracle_mod.OracleError) as e:
+        ref.eval(x, True, "exact")
+    assert e.value.code == 2
+    with pytest.raises(_native.DomainError):
+        dev.eval_values(x, True, "exact")
 
 
 def test_error_semantics_and_recovery():
@@ -195,7 +209,7 @@ def test_error_semantics_and_recovery():
     bad = VolumetricFemEnergy(w.num_vertices, w.tets, w.rest, np.vstack([w.theta[:-1], [[-1.0, 1.0]]]))
     with pytest.raises(ValueError, match="lambda"):
         bad.eval_values(w.u)
-    with pytest.raises(ValueError, match="Hessian kind"):
+    with pytest.raises(ValueError, match="HessianKind|Hessian kind"):
         dev.eval_values(w.u, False, 2)
     again = dev.eval_values(w.u, True, "exact")  # the object stays usable
     assert again[0] == good[0] and np.array_equal(again[1], good[1]) and np.array_equal(again[2], good[2])
