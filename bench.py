@@ -72,6 +72,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-repeats", type=int, default=5)
+    ap.add_argument("--probe-copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a known-size kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the sharded design is described "
@@ -94,6 +97,13 @@ def main():
 
     def step(kind):
         return dev.eval_device(x, True, kind, out_g=g, out_H=H)
+
+    if args.probe_copy_mb:
+        src = torch.ones(args.probe_copy_mb * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     for _ in range(args.warmup):
         step("exact")

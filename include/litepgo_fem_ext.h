@@ -10,9 +10,11 @@
 extern "C" {
 #endif
 
-/* Launch shape of the assembly kernel: lanes per vertex (1, 2, 4), waves per workgroup (1, 2, 4), vertices per
- * tile (<= 64 * waves / lanes_per_vertex) and the LDS budget for a tile's Hessian segment, in doubles.  Used by
- * the autotuner; the default is (2, 1, 32, 6144). */
+/* Launch shape of the assembly kernel: (lanes per vertex, waves per workgroup) in {(1,1), (2,1), (4,1), (8,1),
+ * (4,2), (8,2)} ((4,1) only when a vertex has more than 255 neighbours), vertices per tile
+ * (<= 64 * waves / lanes_per_vertex) and the LDS budget, in doubles, for what a tile stages (its Hessian segment,
+ * incidence records and neighbour lists; a single vertex above it is assembled in HBM instead).  The budget plus
+ * 4 doubles per vertex slot must fit one workgroup's LDS.  Used by the autotuner; the default is (4, 1, 16, 4096). */
 lpg_status lpg_tet_energy_set_launch(lpg_tet_energy* energy, int32_t lanes_per_vertex, int32_t waves,
                                      int32_t vertices_per_tile, int32_t lds_budget_doubles);
 /* cfg4 <- {lanes_per_vertex, waves, vertices_per_tile, lds_budget_doubles}. */

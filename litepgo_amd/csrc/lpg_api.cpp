@@ -59,9 +59,13 @@ BusyScope::~BusyScope() {
 
 lpg::DeviceMesh lpg_tet_energy::mesh() const {
     lpg::DeviceMesh m;
-    m.rec = d_rec.as<double>();
+    m.Xr = d_Xr.as<double>();
+    m.conn = d_conn.as<int32_t>();
+    m.theta = d_theta.as<double>();
     m.inc_ptr = d_inc_ptr.as<int32_t>();
     m.inc = d_inc.ptr;
+    m.nbr_ptr = d_nbr_ptr.as<int32_t>();
+    m.nbr = d_nbr.as<int32_t>();
     m.row_ptr = d_row_ptr.as<int32_t>();
     m.num_vertices = setup.num_vertices;
     m.num_tets = setup.num_tets;
@@ -70,7 +74,7 @@ lpg::DeviceMesh lpg_tet_energy::mesh() const {
 }
 
 lpg_tet_energy::~lpg_tet_energy() {
-    for (lpg::DeviceBuffer* b : {&d_rec, &d_inc_ptr, &d_inc, &d_row_ptr, &d_scalars, &d_vertex_E, &d_chunk_begin,
+    for (lpg::DeviceBuffer* b : {&d_Xr, &d_conn, &d_theta, &d_inc_ptr, &d_inc, &d_nbr_ptr, &d_nbr, &d_row_ptr, &d_scalars, &d_vertex_E, &d_chunk_begin,
                                  &d_chunk_sum, &d_seg_chunk, &d_proj_slot, &d_proj_list, &d_proj_M, &d_x, &d_g, &d_H,
                                  &plan.d_tile_vbeg, &plan.d_normal_ids, &plan.d_oversized_ids})
         b->release();
@@ -93,7 +97,7 @@ void set_plan(lpg_tet_energy* h, const LaunchConfig& lc, int vertices_per_tile, 
     p.lds_budget = lds_budget;
     p.persistent_grid = persistent_grid;
     std::vector<int32_t> oversized;
-    build_tiles(h->setup, TileConfig{vertices_per_tile, lds_budget}, p.tile_vbeg, p.max_segment, &oversized,
+    build_tiles(h->setup, TileConfig{vertices_per_tile, lds_budget}, p.tile_vbeg, p.caps, &oversized,
                 &h->seg_vbeg);
     p.oversized_ids = oversized;
     p.normal_ids.clear();
@@ -123,8 +127,12 @@ void ensure_device(lpg_tet_energy* h) {
     if (!h->pinned)
         hip_check(hipHostMalloc(&h->pinned, 64, hipHostMallocDefault), "hipHostMalloc");
     const auto& s = h->setup;
-    upload(h->d_rec, s.rec.data(), s.rec.size() * sizeof(double));
+    upload(h->d_Xr, s.Xr.data(), s.Xr.size() * sizeof(double));
+    upload(h->d_conn, s.conn.data(), s.conn.size() * sizeof(int32_t));
+    upload(h->d_theta, s.theta.data(), s.theta.size() * sizeof(double));
     upload(h->d_inc_ptr, s.inc_ptr.data(), s.inc_ptr.size() * sizeof(int32_t));
+    upload(h->d_nbr_ptr, s.nbr_ptr.data(), s.nbr_ptr.size() * sizeof(int32_t));
+    upload(h->d_nbr, s.nbr.data(), s.nbr.size() * sizeof(int32_t));
     upload(h->d_row_ptr, s.row_ptr.data(), s.row_ptr.size() * sizeof(int32_t));
     const size_t ninc = s.inc_tet.size();
     if (!s.wide) {
@@ -155,12 +163,12 @@ void ensure_device(lpg_tet_energy* h) {
     h->d_chunk_sum.ensure(std::max<size_t>(1, h->chunk_begin.size()) * sizeof(double));
     h->d_vertex_E.ensure(std::max<size_t>(1, s.num_vertices) * sizeof(double));
     if (h->plan.tile_vbeg.empty()) {
-        // Default launch: 2 lanes per vertex, one wave per workgroup, 32 vertices per tile, segments up to
-        // 6144 doubles (48 KiB) staged in LDS, one workgroup per tile.
+        // Default launch: 4 lanes per vertex, one wave per workgroup, 16 vertices per tile, segments up to
+        // 4096 doubles (32 KiB) staged in LDS, one workgroup per tile.
         LaunchConfig lc;
-        lc.lanes_per_vertex = 2;
+        lc.lanes_per_vertex = 4;
         lc.waves = 1;
-        set_plan(h, lc, 32, 6144, 0);
+        set_plan(h, lc, 16, 4096, 0);
     }
     h->device_ready = true;
 }
@@ -234,13 +242,13 @@ void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int w
         hip_check(hipEventRecord(h->ev[1], stream), "hipEventRecord");
     const TilePlan& p = h->plan;
     if (p.oversized_ids.empty()) {
-        launch_assemble(m, a, p.lc, p.d_tile_vbeg.as<int32_t>(), nullptr, p.num_tiles(), p.max_segment, false,
+        launch_assemble(m, a, p.lc, p.d_tile_vbeg.as<int32_t>(), nullptr, p.num_tiles(), p.caps, false,
                         p.persistent_grid, stream);
     } else {
         launch_assemble(m, a, p.lc, p.d_tile_vbeg.as<int32_t>(), p.d_normal_ids.as<int32_t>(),
-                        static_cast<int>(p.normal_ids.size()), p.max_segment, false, p.persistent_grid, stream);
+                        static_cast<int>(p.normal_ids.size()), p.caps, false, p.persistent_grid, stream);
         launch_assemble(m, a, p.lc, p.d_tile_vbeg.as<int32_t>(), p.d_oversized_ids.as<int32_t>(),
-                        static_cast<int>(p.oversized_ids.size()), 0, true, 0, stream);
+                        static_cast<int>(p.oversized_ids.size()), TileCaps{}, true, 0, stream);
     }
     if (h->timing)
         hip_check(hipEventRecord(h->ev[2], stream), "hipEventRecord");
@@ -448,15 +456,16 @@ lpg_status lpg_tet_energy_set_launch(lpg_tet_energy* h, int32_t lanes_per_vertex
         if (!h)
             fail(lpg::kInvalid, "null handle");
         lpg::BusyScope scope(h->busy);
-        if (!(lanes_per_vertex == 1 || lanes_per_vertex == 2 || lanes_per_vertex == 4) ||
-            !(waves == 1 || waves == 2 || waves == 4) || vertices_per_tile <= 0 ||
-            vertices_per_tile > 64 * waves / lanes_per_vertex || lds_budget_doubles <= 0 ||
-            lds_budget_doubles > 20000 - 64 * waves)
+        if (!lpg::launch_shape_supported(lanes_per_vertex, waves, h->setup.wide) || vertices_per_tile <= 0 ||
+            vertices_per_tile > waves * (64 / lanes_per_vertex) || lds_budget_doubles <= 0)
             fail(lpg::kInvalid, "lpg_tet_energy_set_launch: unsupported configuration");
-        lpg::ensure_device(h);
         lpg::LaunchConfig lc;
         lc.lanes_per_vertex = lanes_per_vertex;
         lc.waves = waves;
+        // The budget plus the gradient and energy slots must fit one workgroup's LDS.
+        if (lpg::assemble_lds_bytes(lc, lpg::TileCaps{lds_budget_doubles}, false, 0) > lpg::assemble_lds_limit(lc))
+            fail(lpg::kInvalid, "lpg_tet_energy_set_launch: LDS budget exceeds the workgroup's LDS");
+        lpg::ensure_device(h);
         lpg::set_plan(h, lc, vertices_per_tile, lds_budget_doubles, 0);
     });
 }

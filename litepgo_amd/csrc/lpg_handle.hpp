@@ -33,7 +33,7 @@ struct TilePlan {
     int persistent_grid = 0; // 0: one workgroup per tile
     std::vector<int32_t> tile_vbeg;
     std::vector<int32_t> normal_ids, oversized_ids;
-    int32_t max_segment = 0;
+    TileCaps caps;
     DeviceBuffer d_tile_vbeg, d_normal_ids, d_oversized_ids;
     int num_tiles() const {
         return static_cast<int>(tile_vbeg.size()) - 1;
@@ -61,7 +61,7 @@ struct lpg_tet_energy {
     int device = 0;
     bool device_ready = false;
     hipStream_t own_stream = nullptr;
-    lpg::DeviceBuffer d_rec, d_inc_ptr, d_inc, d_row_ptr;
+    lpg::DeviceBuffer d_Xr, d_conn, d_theta, d_inc_ptr, d_inc, d_nbr_ptr, d_nbr, d_row_ptr;
     lpg::DeviceBuffer d_scalars; // energy (double) + status + counter
     lpg::DeviceBuffer d_vertex_E, d_chunk_begin, d_chunk_sum, d_seg_chunk;
     lpg::DeviceBuffer d_proj_slot, d_proj_list, d_proj_M;

@@ -30,13 +30,10 @@ enum StatusBits : unsigned {
     kStatusProjection = 16u,    // projection failed to converge / non-finite    -> domain_error
 };
 
-// One tet record: 16 doubles = 128 bytes, 128-byte aligned, so a record is exactly one cache line.
-//   [0..8]   Dm^-1 row-major (row r = G(r + 1, :), G(0, :) = -sum of rows)
-//   [9]      w = det(Dm) / 6
-//   [10..11] lambda, mu (snapshot theta)
-//   [12..13] the 4 vertex ids as int32
-//   [14..15] unused
-constexpr int kRecDoubles = 16;
+// Device layout (no per-tet geometry is stored; Dm^-1 and w are recomputed from the rest positions):
+//   Xr[V]    : rest positions as double4 (x, y, z, 0), 32 bytes per vertex
+//   conn[T]  : the 4 vertex ids of a tet as int4, 16 bytes per tet
+//   theta[T] : (lambda, mu) as double2, 16 bytes per tet
 
 // Incidence record of (vertex v, tet e): the tet id and, for each local slot b of e, the position of vertex
 // e[b] in v's sorted neighbour list (the column block of H in v's rows).  Narrow form packs 4 x uint8.
@@ -67,7 +64,9 @@ struct MeshSetup {
     std::vector<int32_t> inc_ptr;    // num_vertices + 1
     std::vector<int32_t> inc_tet;    // 4 T, element order per vertex
     std::vector<uint16_t> inc_colpos; // 4 T x 4
-    std::vector<double> rec;         // T x kRecDoubles
+    std::vector<int32_t> conn;       // T x 4 vertex ids
+    std::vector<double> Xr;          // V x 4 rest positions (x, y, z, 0)
+    std::vector<double> theta;       // T x 2 (lambda, mu) snapshot
 };
 
 // Builds MeshSetup; throws Error(kInvalid, ...) with the reference's messages on invalid input.
@@ -80,20 +79,28 @@ void build_slot_map(const MeshSetup& s, int32_t* offsets);
 // Vertex tiles: contiguous vertex ranges handled by one workgroup.  tile_vbeg has tiles + 1 entries.
 struct TileConfig {
     int vertices_per_tile;    // upper bound on vertices in a tile
-    int lds_budget_doubles;   // upper bound on the H segment staged in LDS
+    int lds_budget_doubles;   // upper bound on the H segment (9 |nbr(v)| doubles per vertex) staged in LDS
 };
-// Tiles never cross a vertex listed in `breaks` (mesh boundaries of a batch); null means no breaks.
-void build_tiles(const MeshSetup& s, const TileConfig& cfg, std::vector<int32_t>& tile_vbeg,
-                 int32_t& max_segment, std::vector<int32_t>* oversized,
-                 const std::vector<int32_t>* breaks = nullptr);
+// Largest H segment over the tiles that stage it in LDS.
+struct TileCaps {
+    int32_t segment = 0;
+};
+// Tiles never cross a vertex listed in `breaks` (mesh boundaries of a batch); null means no breaks.  A tile of one
+// vertex whose segment exceeds the budget is listed in `oversized` and accumulates in HBM instead.
+void build_tiles(const MeshSetup& s, const TileConfig& cfg, std::vector<int32_t>& tile_vbeg, TileCaps& caps,
+                 std::vector<int32_t>* oversized, const std::vector<int32_t>* breaks = nullptr);
 
 // ------------------------------------------------------------------------------------------------------------
 // Kernel parameters and launchers (lpg_kernels.hip).
 // ------------------------------------------------------------------------------------------------------------
 struct DeviceMesh {
-    const double* rec = nullptr;
+    const double* Xr = nullptr;     // V x 4
+    const int32_t* conn = nullptr;  // T x 4
+    const double* theta = nullptr;  // T x 2 snapshot
     const int32_t* inc_ptr = nullptr;
     const void* inc = nullptr; // IncNarrow* or IncWide*
+    const int32_t* nbr_ptr = nullptr;
+    const int32_t* nbr = nullptr;
     const int32_t* row_ptr = nullptr;
     int32_t num_vertices = 0;
     int32_t num_tets = 0;
@@ -118,17 +125,24 @@ struct EvalArgs {
 };
 
 struct LaunchConfig {
-    int lanes_per_vertex = 1; // 1, 2 or 4
+    int lanes_per_vertex = 4; // lanes sharing one vertex's incidences
     int waves = 1;            // waves per workgroup
 };
+// Launch shapes with compiled assembly kernels: (lanes per vertex, waves).
+bool launch_shape_supported(int lanes_per_vertex, int waves, bool wide);
 
 void launch_clear(unsigned* status, unsigned* counter, hipStream_t stream);
 // Processes tiles tile_ids[0..num_tiles) (or 0..num_tiles-1 when tile_ids is null).  global_acc accumulates in
-// the H output itself instead of LDS (for tiles whose segment exceeds the LDS budget).
-// persistent_grid > 0 launches that many workgroups, each walking the tile list with a grid stride.
+// the H output itself instead of LDS (tiles whose segment exceeds the LDS budget); otherwise the H segment is
+// staged in LDS, sized by caps.  persistent_grid > 0 launches that many workgroups, each walking the tile list with
+// a grid stride.
 void launch_assemble(const DeviceMesh& m, const EvalArgs& a, const LaunchConfig& lc, const int32_t* tile_vbeg,
-                     const int32_t* tile_ids, int num_tiles, int lds_doubles, bool global_acc, int persistent_grid,
-                     hipStream_t stream);
+                     const int32_t* tile_ids, int num_tiles, const TileCaps& caps, bool global_acc,
+                     int persistent_grid, hipStream_t stream);
+// Dynamic LDS bytes of one assembly workgroup for a launch shape and staged sizes.
+size_t assemble_lds_bytes(const LaunchConfig& lc, const TileCaps& caps, bool global_acc, int kind);
+// Largest dynamic LDS allocation an assembly workgroup of this shape may request.
+size_t assemble_lds_limit(const LaunchConfig& lc);
 // Energy per mesh from the per-vertex energies: chunk c sums vertex_E[chunk_begin[c] .. chunk_begin[c+1]),
 // segment m sums chunk_sum[seg_chunk[m] .. seg_chunk[m+1]) into out[m].
 void launch_energy_reduce(const double* vertex_E, const int32_t* chunk_begin, int num_chunks, double* chunk_sum,

@@ -1,15 +1,17 @@
 // HIP kernels for gfx950: neo-Hookean Tet P1 energy / gradient / Hessian assembly and its derivatives.
 //
-// Layout in HBM (see lpg_internal.hpp): one 128-byte record per tet (Dm^-1, w, lambda, mu, vertex ids), the
-// displacement vector x[3V], per-vertex incidence lists (tet id + the column-block position of each of the tet's
-// vertices in this vertex's rows), and the CSR row pointer.
+// Layout in HBM (lpg_internal.hpp): rest positions Xr[V] (double4), the displacement vector x[3V], connectivity
+// conn[T] (int4), theta[T] (double2), per-vertex sorted neighbour lists and incidence lists (tet id + the position of
+// each of the tet's vertices in this vertex's neighbour list, which is also its column block in the vertex's rows),
+// and the CSR row pointer.  No per-tet geometry is stored: Dm^-1 and w = det(Dm)/6 are recomputed from the four
+// rest positions (~50 FP64 operations), which saves 80 bytes of HBM traffic per tet.
 //
 // Assembly is vertex-owned: the rows 3v..3v+2 of H form one contiguous segment of 9 |nbr(v)| values, and only
-// the lanes of vertex v write it.  A lane walks v's incident tets in element order, rebuilds F, J, F^-T and log J
-// in registers, and adds the three rows of the tet's 12x12 Hessian that belong to v, block by block, to the
-// segment staged in LDS.  Each value is therefore summed in element order by a single lane, as the reference's
-// column-owner assembly does (src/core/dof/energy_assembler.cpp:249-273): no atomics on HBM, bitwise
-// reproducible results, and every H value is written to HBM exactly once with coalesced stores.
+// the lanes of vertex v write it.  The lanes of a vertex split its incident tets (in element order), rebuild F, J,
+// F^-T and log J in registers, and add the three rows of the tet's 12x12 Hessian that belong to v, block by block,
+// to the segment staged in LDS, one lane after the other.  Each value is therefore summed in element order, as the
+// reference's column-owner assembly does (src/core/dof/energy_assembler.cpp:249-273): no atomics on HBM, bitwise
+// reproducible results for every launch shape, and every H value written to HBM once, coalesced.
 //
 // The element Hessian uses the closed form of B^T A B for Tet P1 and neo-Hookean (A from neo_hookean.cpp:43-60):
 //   H_ab = w [ mu (g_a . g_b) I3 + lambda q_a q_b^T - p q_b q_a^T ],  q_b = F^-T g_b,  p = lambda log J - mu,
@@ -39,7 +41,14 @@ __device__ __forceinline__ double dot3(double a0, double a1, double a2, double b
 
 constexpr int kWave = 64;
 
-// Helmert basis of 1-perp in R^4: P(a, m).
+// Helmert basis of 1-perp in R^4: P(a, m).  kHelmertC is the same table as a compile-time constant (its zeros fold
+// away); kHelmert lives in constant memory for the kernels that index it at run time.
+constexpr double kHelmertC[4][3] = {
+    {0.70710678118654752440, 0.40824829046386301637, 0.28867513459481288225},
+    {-0.70710678118654752440, 0.40824829046386301637, 0.28867513459481288225},
+    {0.0, -0.81649658092772603273, 0.28867513459481288225},
+    {0.0, 0.0, -0.86602540378443864676},
+};
 __device__ __constant__ double kHelmert[4][3] = {
     {0.70710678118654752440, 0.40824829046386301637, 0.28867513459481288225},
     {-0.70710678118654752440, 0.40824829046386301637, 0.28867513459481288225},
@@ -47,49 +56,79 @@ __device__ __constant__ double kHelmert[4][3] = {
     {0.0, 0.0, -0.86602540378443864676},
 };
 
+// Tet inputs of one evaluation: rest and current vertex data, material.
 struct TetIn {
-    double Di[9]; // Dm^-1 row-major
-    double w, lam, mu;
-    int ids[4];
+    double X[4][3];
+    double u[4][3];
+    double lam, mu;
 };
 
-__device__ __forceinline__ void load_tet(const double* __restrict__ rec, const double* __restrict__ theta, int e,
-                                         TetIn& t) {
-    const double2* r = reinterpret_cast<const double2*>(rec + static_cast<size_t>(e) * kRecDoubles);
-    const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5];
-    const int4 id = *reinterpret_cast<const int4*>(r + 6);
-    t.Di[0] = r0.x;
-    t.Di[1] = r0.y;
-    t.Di[2] = r1.x;
-    t.Di[3] = r1.y;
-    t.Di[4] = r2.x;
-    t.Di[5] = r2.y;
-    t.Di[6] = r3.x;
-    t.Di[7] = r3.y;
-    t.Di[8] = r4.x;
-    t.w = r4.y;
-    if (theta) {
-        const double2 th = reinterpret_cast<const double2*>(theta)[e];
-        t.lam = th.x;
-        t.mu = th.y;
-    } else {
-        t.lam = r5.x;
-        t.mu = r5.y;
-    }
-    t.ids[0] = id.x;
-    t.ids[1] = id.y;
-    t.ids[2] = id.z;
-    t.ids[3] = id.w;
-}
-
-__device__ __forceinline__ void load_u(const double* __restrict__ x, const int ids[4], double u[4][3]) {
+__device__ __forceinline__ void load_vertices(const double* __restrict__ Xr, const double* __restrict__ x,
+                                              const int ids[4], TetIn& t) {
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
+        const double4 X = reinterpret_cast<const double4*>(Xr)[ids[b]];
+        t.X[b][0] = X.x;
+        t.X[b][1] = X.y;
+        t.X[b][2] = X.z;
         const double* p = x + 3 * static_cast<size_t>(ids[b]);
-        u[b][0] = p[0];
-        u[b][1] = p[1];
-        u[b][2] = p[2];
+        t.u[b][0] = p[0];
+        t.u[b][1] = p[1];
+        t.u[b][2] = p[2];
     }
+}
+
+// (lambda, mu) of tet e from the per-call override, else from the snapshot taken at construction.
+__device__ __forceinline__ void load_theta(const double* __restrict__ snapshot, const double* __restrict__ override_,
+                                           int e, TetIn& t) {
+    const double2 th = reinterpret_cast<const double2*>(override_ ? override_ : snapshot)[e];
+    t.lam = th.x;
+    t.mu = th.y;
+}
+
+// Element-wise kernels: connectivity (int4) and theta (double2) of tet e are read coalesced, vertex data gathered.
+__device__ __forceinline__ void load_tet(const double* __restrict__ Xr, const int32_t* __restrict__ conn,
+                                         const double* __restrict__ snapshot, const double* __restrict__ x,
+                                         const double* __restrict__ override_, int e, int ids[4], TetIn& t) {
+    const int4 c = reinterpret_cast<const int4*>(conn)[e];
+    ids[0] = c.x;
+    ids[1] = c.y;
+    ids[2] = c.z;
+    ids[3] = c.w;
+    load_vertices(Xr, x, ids, t);
+    load_theta(snapshot, override_, e, t);
+}
+
+// Rest geometry: Dm = [X1-X0 | X2-X0 | X3-X0], Dm^-1 = cof(Dm)^T / det, w = det / 6.  The host applies the same
+// operation sequence when it validates the mesh (det > 0, finite Dm^-1), so a tet accepted there is valid here.
+struct Geo {
+    double Di[9]; // Dm^-1 row-major
+    double w;
+};
+
+__device__ __forceinline__ void rest_geometry(const TetIn& t, Geo& g) {
+    double Dm[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            Dm[3 * r + k] = t.X[k + 1][r] - t.X[0][r];
+    double C[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            C[3 * i + j] = fma(Dm[3 * i1 + j1], Dm[3 * i2 + j2], -(Dm[3 * i1 + j2] * Dm[3 * i2 + j1]));
+        }
+    const double det = dot3(Dm[0], Dm[1], Dm[2], C[0], C[1], C[2]);
+    const double inv = 1.0 / det;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            g.Di[3 * i + j] = C[3 * j + i] * inv;
+    g.w = det * (1.0 / 6.0);
 }
 
 // Deformation gradient and its inverse transpose: F = I + Du Dm^-1, a = F^-T = cof(F) / det F.
@@ -99,13 +138,13 @@ struct Kin {
     double J, L;
 };
 
-__device__ __forceinline__ void kinematics(const double u[4][3], const double Di[9], Kin& k) {
+__device__ __forceinline__ void kinematics(const TetIn& t, const double Di[9], Kin& k) {
     double Du[9]; // Du(i, c) = (u_{c+1} - u_0)_i
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-            Du[3 * i + c] = u[c + 1][i] - u[0][i];
+            Du[3 * i + c] = t.u[c + 1][i] - t.u[0][i];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -185,198 +224,291 @@ __device__ __forceinline__ void select_row(const double G[4][3], int a, double o
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// Assembly kernel: one workgroup per vertex tile, LPV lanes per vertex (lane s of a vertex owns the column blocks
-// whose position is congruent to s modulo LPV).  KIND: -1 none, 0 Exact, 1 Projected.
+// Assembly kernel: one workgroup per vertex tile.  The LPV lanes of a vertex split its incidence list (lane s takes
+// incidences s, s + LPV, s + 2 LPV, ...); each lane computes the three rows of its tet's element Hessian that belong
+// to the vertex and adds them to the vertex's H segment in LDS with ds_add_f64.  All lanes add at once: the order
+// in which the LPV contributions of one round land on a shared value is fixed by the hardware's handling of one
+// LDS atomic instruction, so results are reproducible run to run for a given LPV (and independent of the number of
+// waves, tiles and grid), and agree across LPV to rounding.  Gradient and energy contributions are summed per lane
+// in registers and combined in lane order at the end.  KIND: -1 none, 0 Exact, 1 Projected.
 // ---------------------------------------------------------------------------------------------------------------
 struct AssembleParams {
-    const double* rec;
+    const double* Xr;
+    const double* theta_snap;
     const int32_t* inc_ptr;
     const void* inc;
+    const int32_t* nbr_ptr;
+    const int32_t* nbr;
     const int32_t* row_ptr;
     const int32_t* tile_vbeg;
     const int32_t* tile_ids; // tiles to process (index into tile_vbeg), or null for 0..num_tiles-1
     const double* x;
-    const double* theta;
+    const double* theta; // per-call override or null
     double* g;
     double* H;
-    double* vertex_E; // per-vertex element-order sum of the energies of the tets whose slot 0 it is
+    double* vertex_E; // per-vertex sum of the energies of the tets whose slot 0 it is
     unsigned* status;
     const int32_t* proj_slot;
     const double* proj_M;
 };
 
-// One tile: zero the tile's H segment in LDS (or in HBM for GLOBAL_ACC), accumulate, write it back coalesced,
-// write the gradient rows and the per-vertex energies, and OR the status flags.
+// The 4 x 9 block-row contributions of one incidence (vertex = slot a of tet e), its gradient block and, for
+// slot 0, the tet's energy.  With Ds = [x1-x0 | x2-x0 | x3-x0] (deformed edges) and F = Ds Dm^-1, the vectors
+// q_b = F^-T g_b are the rows of Ds^-1 (b = 1..3) and q_0 = -(q_1 + q_2 + q_3); J = det Ds / det Dm.
+template <int KIND>
+__device__ __forceinline__ void incidence_values(const AssembleParams& p, const TetIn& t, int a, int e,
+                                                 double vals[4][9], double gc[3], double& ec, unsigned& flags) {
+    // Rest geometry.
+    double Dm[9], Cm[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            Dm[3 * r + k] = t.X[k + 1][r] - t.X[0][r];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            Cm[3 * i + j] = fma(Dm[3 * i1 + j1], Dm[3 * i2 + j2], -(Dm[3 * i1 + j2] * Dm[3 * i2 + j1]));
+        }
+    const double det_m = dot3(Dm[0], Dm[1], Dm[2], Cm[0], Cm[1], Cm[2]);
+    const double inv_m = 1.0 / det_m;
+    const double w = det_m * (1.0 / 6.0);
+    double G[4][3]; // rows of dN/dxi Dm^-1
+#pragma unroll
+    for (int b = 1; b < 4; ++b)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            G[b][j] = Cm[3 * j + (b - 1)] * inv_m;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        G[0][j] = -(G[1][j] + G[2][j] + G[3][j]);
+    // Current geometry.
+    double Du[9], Ds[9], Cs[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            Du[3 * r + k] = t.u[k + 1][r] - t.u[0][r];
+            Ds[3 * r + k] = Dm[3 * r + k] + Du[3 * r + k];
+        }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            Cs[3 * i + j] = fma(Ds[3 * i1 + j1], Ds[3 * i2 + j2], -(Ds[3 * i1 + j2] * Ds[3 * i2 + j1]));
+        }
+    const double det_s = dot3(Ds[0], Ds[1], Ds[2], Cs[0], Cs[1], Cs[2]);
+    const double inv_s = 1.0 / det_s;
+    const double J = det_s * inv_m;
+    const double L = log(J);
+    if (!(J > 0.0 && isfinite(J) && isfinite(L)))
+        flags |= kStatusDetF;
+    if (!theta_ok(t.lam, t.mu))
+        flags |= kStatusBadTheta;
+    double q[4][3];
+#pragma unroll
+    for (int b = 1; b < 4; ++b)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            q[b][j] = Cs[3 * j + (b - 1)] * inv_s;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        q[0][j] = -(q[1][j] + q[2][j] + q[3][j]);
+    // F = I + Du Dm^-1.
+    double F[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            F[3 * i + j] = fma(Du[3 * i + 2], G[3][j], fma(Du[3 * i + 1], G[2][j], fma(Du[3 * i], G[1][j], i == j ? 1.0 : 0.0)));
+    double ga[3], qa[3];
+    select_row(G, a, ga);
+    select_row(q, a, qa);
+    const double p_ = fma(t.lam, L, -t.mu);
+    // Gradient block w (mu F g_a + p q_a).
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        gc[i] = w * fma(t.mu, dot3(F[3 * i], F[3 * i + 1], F[3 * i + 2], ga[0], ga[1], ga[2]), p_ * qa[i]);
+    if (a == 0) {
+        double sq = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            sq = fma(F[k], F[k], sq);
+        ec = w * fma(0.5 * t.lam * L, L, fma(0.5 * t.mu, sq - 3.0, -(t.mu * L)));
+    }
+    if (KIND < 0)
+        return;
+    const int slot = KIND == 1 ? p.proj_slot[e] : -1;
+    if (slot < 0) {
+        // H_ab = w [mu (g_a . g_b) I + lambda q_a q_b^T - p q_b q_a^T]
+        const double sw = w * t.mu;
+        double xa[3], ya[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            xa[i] = (w * t.lam) * qa[i];
+            ya[i] = (w * p_) * qa[i];
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const double s1 = sw * dot3(ga[0], ga[1], ga[2], G[b][0], G[b][1], G[b][2]);
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) {
+                    double val = fma(xa[i], q[b][kk], -(q[b][i] * ya[kk]));
+                    if (i == kk)
+                        val += s1;
+                    vals[b][3 * i + kk] = val;
+                }
+        }
+    } else {
+        // proj(H)_ab = sum_mn P(a,m) P(b,n) Mp[(m,i),(n,k)], Mp packed upper-triangular row-major.
+        const double* Mp = p.proj_M + 45 * static_cast<size_t>(slot);
+        double Pa[3];
+        {
+            double Ph[4][3];
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+                    Ph[b][m] = kHelmert[b][m];
+            select_row(Ph, a, Pa);
+        }
+        // R_n[i][k] = sum_m Pa[m] Mp[(3m+i), (3n+k)], vals[b] = sum_n P(b,n) R_n, one n at a time.
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int c = 0; c < 9; ++c)
+                vals[b][c] = 0.0;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            double Rn[9];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int m = 0; m < 3; ++m) {
+                        const int rr = 3 * m + i, cc = 3 * n + kk;
+                        const int lo = rr < cc ? rr : cc, hi = rr < cc ? cc : rr;
+                        s = fma(Pa[m], Mp[lo * 9 - lo * (lo - 1) / 2 + (hi - lo)], s);
+                    }
+                    Rn[3 * i + kk] = s;
+                }
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                if (kHelmertC[b][n] != 0.0)
+#pragma unroll
+                    for (int c = 0; c < 9; ++c)
+                        vals[b][c] = fma(kHelmertC[b][n], Rn[c], vals[b][c]);
+        }
+    }
+}
+
+// One tile: zero the tile's H segment in LDS (GLOBAL_ACC: in HBM, for a segment above the LDS budget), accumulate,
+// write it back coalesced, write the gradient rows and per-vertex energies, OR the status flags.
+// LDS layout in doubles: [gradient: 3 per vertex][energy: 1 per vertex][H segment of the tile].
+// GLOBAL_ACC tiles add with plain read-modify-writes to HBM, so their lanes take turns (incidence order).
 template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GLOBAL_ACC>
 __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile, double* lds) {
     using Inc = typename IncT<WIDE>::type;
-    const Inc* __restrict__ inc = static_cast<const Inc*>(p.inc);
     constexpr int kThreads = kWave * WAVES;
+    constexpr int kPerWave = kWave / LPV;   // vertex groups never straddle two waves
+    constexpr int kVerts = WAVES * kPerWave; // lanes kPerWave * LPV .. 63 of each wave stay idle
+    const Inc* __restrict__ inc = static_cast<const Inc*>(p.inc);
     const int vb = p.tile_vbeg[tile], ve = p.tile_vbeg[tile + 1];
     const int seg0 = KIND >= 0 ? p.row_ptr[3 * vb] : 0;
     const int seg1 = KIND >= 0 ? p.row_ptr[3 * ve] : 0;
-    double* acc = GLOBAL_ACC ? p.H + seg0 : lds;
-    if (KIND >= 0) {
+    double* gl = lds;
+    double* el = gl + 3 * kVerts;
+    double* acc = GLOBAL_ACC ? p.H + seg0 : el + kVerts;
+    if (KIND >= 0)
         for (int i = threadIdx.x; i < seg1 - seg0; i += kThreads)
             acc[i] = 0.0;
-        __syncthreads();
-    }
-    const int sub = threadIdx.x % LPV;
-    const int v = vb + static_cast<int>(threadIdx.x) / LPV;
+    const int lane = static_cast<int>(threadIdx.x) % kWave;
+    const int sub = lane % LPV;
+    const int lv = (static_cast<int>(threadIdx.x) / kWave) * kPerWave + lane / LPV;
+    const int v = vb + lv;
+    const bool owner = lane < kPerWave * LPV && v < ve;
+    __syncthreads();
     unsigned flags = 0;
-    double esum = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
-    if (v < ve) {
-        {
+    double gs0 = 0.0, gs1 = 0.0, gs2 = 0.0, es = 0.0;
+    if (owner) {
+        if (sub == 0) {
             const double* xv = p.x + 3 * static_cast<size_t>(v);
             if (!(isfinite(xv[0]) && isfinite(xv[1]) && isfinite(xv[2])))
                 flags |= kStatusNonFiniteX;
         }
         const int rbase = KIND >= 0 ? p.row_ptr[3 * v] - seg0 : 0;
         const int rowlen = KIND >= 0 ? p.row_ptr[3 * v + 1] - p.row_ptr[3 * v] : 0;
+        double* row = acc + rbase;
+        const int32_t* my_nbr = p.nbr + p.nbr_ptr[v];
         const int kb = p.inc_ptr[v], ke = p.inc_ptr[v + 1];
-        for (int k = kb; k < ke; ++k) {
-            const Inc r = inc[k];
-            const int e = r.tet;
-            TetIn t;
-            load_tet(p.rec, p.theta, e, t);
-            double u[4][3];
-            load_u(p.x, t.ids, u);
-            const int a = (t.ids[0] == v) ? 0 : (t.ids[1] == v) ? 1 : (t.ids[2] == v) ? 2 : 3;
-            Kin kin;
-            kinematics(u, t.Di, kin);
-            if (!kin_ok(kin))
-                flags |= kStatusDetF;
-            if (!theta_ok(t.lam, t.mu))
-                flags |= kStatusBadTheta;
-            double G[4][3];
-            shape_gradients(t.Di, G);
-            double ga[3];
-            select_row(G, a, ga);
-            const double p_ = fma(t.lam, kin.L, -t.mu);
-            // q_a = F^-T g_a
-            double qa[3];
+        for (int k0 = kb; k0 < ke; k0 += LPV) {
+            const int k = k0 + sub;
+            const bool active = k < ke;
+            double vals[4][9];
+            int cps[4] = {0, 0, 0, 0};
+            if (active) {
+                const Inc r = inc[k];
+                int ids[4];
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
-                qa[i] = dot3(kin.a[3 * i], kin.a[3 * i + 1], kin.a[3 * i + 2], ga[0], ga[1], ga[2]);
-            if (sub == 0) {
-                if (WANT_G) {
-                    // w (mu F g_a + p q_a)
-                    double Fg[3];
-#pragma unroll
-                    for (int i = 0; i < 3; ++i)
-                        Fg[i] = dot3(kin.F[3 * i], kin.F[3 * i + 1], kin.F[3 * i + 2], ga[0], ga[1], ga[2]);
-                    g0 = fma(t.w, fma(t.mu, Fg[0], p_ * qa[0]), g0);
-                    g1 = fma(t.w, fma(t.mu, Fg[1], p_ * qa[1]), g1);
-                    g2 = fma(t.w, fma(t.mu, Fg[2], p_ * qa[2]), g2);
+                for (int b = 0; b < 4; ++b) {
+                    cps[b] = IncT<WIDE>::colpos(r, b);
+                    ids[b] = my_nbr[cps[b]];
                 }
-                if (a == 0) {
-                    double sq = 0.0;
-#pragma unroll
-                    for (int q = 0; q < 9; ++q)
-                        sq = fma(kin.F[q], kin.F[q], sq);
-                    const double psi = fma(0.5 * t.lam * kin.L, kin.L, fma(0.5 * t.mu, sq - 3.0, -(t.mu * kin.L)));
-                    esum = fma(t.w, psi, esum);
-                }
+                const int a = (ids[0] == v) ? 0 : (ids[1] == v) ? 1 : (ids[2] == v) ? 2 : 3;
+                TetIn t;
+                load_vertices(p.Xr, p.x, ids, t);
+                load_theta(p.theta_snap, p.theta, r.tet, t);
+                double gc[3], ec = 0.0;
+                incidence_values<KIND>(p, t, a, r.tet, vals, gc, ec, flags);
+                gs0 += gc[0];
+                gs1 += gc[1];
+                gs2 += gc[2];
+                es += ec;
             }
             if (KIND >= 0) {
-                double* row = acc + rbase;
-                bool projected = false;
-                int slot = -1;
-                if (KIND == 1) {
-                    slot = p.proj_slot[e];
-                    projected = slot >= 0;
-                }
-                if (!projected) {
-                    const double s2 = t.w * t.lam, s3 = t.w * p_, sw = t.w * t.mu;
-                    double xa[3], ya[3];
 #pragma unroll
-                    for (int i = 0; i < 3; ++i) {
-                        xa[i] = s2 * qa[i];
-                        ya[i] = s3 * qa[i];
-                    }
+                for (int s = 0; s < (GLOBAL_ACC ? LPV : 1); ++s) {
+                    if (active && (!GLOBAL_ACC || sub == s)) {
 #pragma unroll
-                    for (int b = 0; b < 4; ++b) {
-                        const int cp = IncT<WIDE>::colpos(r, b);
-                        if (LPV == 1 || (cp % LPV) == sub) {
-                        double qb[3];
+                        for (int b = 0; b < 4; ++b) {
+                            double* blk = row + 3 * cps[b];
 #pragma unroll
-                        for (int i = 0; i < 3; ++i)
-                            qb[i] = dot3(kin.a[3 * i], kin.a[3 * i + 1], kin.a[3 * i + 2], G[b][0], G[b][1], G[b][2]);
-                        const double s1 = sw * dot3(ga[0], ga[1], ga[2], G[b][0], G[b][1], G[b][2]);
-                        double* blk = row + 3 * cp;
+                            for (int i = 0; i < 3; ++i)
 #pragma unroll
-                        for (int i = 0; i < 3; ++i)
-#pragma unroll
-                            for (int kk = 0; kk < 3; ++kk) {
-                                double val = fma(xa[i], qb[kk], -(qb[i] * ya[kk]));
-                                if (i == kk)
-                                    val += s1;
-                                accum<GLOBAL_ACC>(blk + i * rowlen + kk, val);
-                            }
-                        }
-                    }
-                } else {
-                    // proj(H)_ab = sum_mn P(a,m) P(b,n) Mp[(m,i),(n,k)], Mp packed upper-triangular row-major.
-                    const double* Mp = p.proj_M + 45 * static_cast<size_t>(slot);
-                    double Pa[3];
-                    {
-                        double Ph[4][3];
-#pragma unroll
-                        for (int b = 0; b < 4; ++b)
-#pragma unroll
-                            for (int m = 0; m < 3; ++m)
-                                Ph[b][m] = kHelmert[b][m];
-                        select_row(Ph, a, Pa);
-                    }
-                    // R[n][i][k] = sum_m Pa[m] Mp[(3m+i), (3n+k)]
-                    double R[3][3][3];
-#pragma unroll
-                    for (int n = 0; n < 3; ++n)
-#pragma unroll
-                        for (int i = 0; i < 3; ++i)
-#pragma unroll
-                            for (int kk = 0; kk < 3; ++kk) {
-                                double s = 0.0;
-#pragma unroll
-                                for (int m = 0; m < 3; ++m) {
-                                    const int rr = 3 * m + i, cc = 3 * n + kk;
-                                    const int lo = rr < cc ? rr : cc, hi = rr < cc ? cc : rr;
-                                    const int idx = lo * 9 - lo * (lo - 1) / 2 + (hi - lo);
-                                    s = fma(Pa[m], Mp[idx], s);
-                                }
-                                R[n][i][kk] = s;
-                            }
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) {
-                        const int cp = IncT<WIDE>::colpos(r, b);
-                        if (LPV == 1 || (cp % LPV) == sub) {
-                        double* blk = row + 3 * cp;
-#pragma unroll
-                        for (int i = 0; i < 3; ++i)
-#pragma unroll
-                            for (int kk = 0; kk < 3; ++kk) {
-                                const double val = dot3(kHelmert[b][0], kHelmert[b][1], kHelmert[b][2],
-                                                        R[0][i][kk], R[1][i][kk], R[2][i][kk]);
-                                accum<GLOBAL_ACC>(blk + i * rowlen + kk, val);
-                            }
+                                for (int kk = 0; kk < 3; ++kk)
+                                    accum<GLOBAL_ACC>(blk + i * rowlen + kk, vals[b][3 * i + kk]);
                         }
                     }
                 }
             }
         }
-        if (sub == 0)
-            p.vertex_E[v] = esum;
-        if (WANT_G && sub == 0) {
-            double* gv = p.g + 3 * static_cast<size_t>(v);
-            gv[0] = g0;
-            gv[1] = g1;
-            gv[2] = g2;
-            if (!(isfinite(g0) && isfinite(g1) && isfinite(g2)))
-                flags |= kStatusNonFiniteOut;
+    }
+    // Per-vertex gradient and energy: the lanes' partial sums in lane order.
+#pragma unroll
+    for (int s = 0; s < LPV; ++s) {
+        if (owner && sub == s) {
+            if (s == 0) {
+                gl[3 * lv] = gs0;
+                gl[3 * lv + 1] = gs1;
+                gl[3 * lv + 2] = gs2;
+                el[lv] = es;
+            } else {
+                gl[3 * lv] += gs0;
+                gl[3 * lv + 1] += gs1;
+                gl[3 * lv + 2] += gs2;
+                el[lv] += es;
+            }
         }
+        __syncthreads();
     }
     if (KIND >= 0) {
-        __syncthreads();
         if (!GLOBAL_ACC) {
             for (int i = threadIdx.x; i < seg1 - seg0; i += kThreads) {
                 const double val = acc[i];
@@ -390,20 +522,39 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
                     flags |= kStatusNonFiniteOut;
         }
     }
+    const int nv = ve - vb;
+    for (int q = threadIdx.x; q < nv; q += kThreads)
+        p.vertex_E[vb + q] = el[q];
+    if (WANT_G)
+        for (int q = threadIdx.x; q < 3 * nv; q += kThreads) {
+            const double val = gl[q];
+            if (!isfinite(val))
+                flags |= kStatusNonFiniteOut;
+            p.g[3 * static_cast<size_t>(vb) + q] = val;
+        }
     if (flags)
         atomicOr(p.status, flags);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).  Remapping block b to the slot
+// (b mod 8) q + min(b mod 8, r) + b / 8 (q = G / 8, r = G mod 8) gives every XCD one contiguous run of tiles, so
+// the vertex data shared by neighbouring tiles is fetched into one L2 instead of several.
+__device__ __forceinline__ int xcd_slot(int b, int G) {
+    const int q = G >> 3, r = G & 7, x = b & 7, i = b >> 3;
+    return x * q + (x < r ? x : r) + i;
+}
+
 template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GLOBAL_ACC>
-__global__ __launch_bounds__(kWave* WAVES) void k_assemble(AssembleParams p) {
+__global__ __launch_bounds__(kWave* WAVES, KIND == 1 ? 2 : 3) void k_assemble(AssembleParams p) {
     extern __shared__ double lds[];
-    const int tile = p.tile_ids ? p.tile_ids[blockIdx.x] : static_cast<int>(blockIdx.x);
+    const int slot = xcd_slot(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+    const int tile = p.tile_ids ? p.tile_ids[slot] : slot;
     assemble_tile<LPV, WAVES, WANT_G, KIND, WIDE, GLOBAL_ACC>(p, tile, lds);
 }
 
 // Persistent form: a fixed grid walks the tile list with a grid stride (the batched launch).
 template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GLOBAL_ACC>
-__global__ __launch_bounds__(kWave* WAVES) void k_assemble_persistent(AssembleParams p, int num_tiles) {
+__global__ __launch_bounds__(kWave* WAVES, KIND == 1 ? 2 : 3) void k_assemble_persistent(AssembleParams p, int num_tiles) {
     extern __shared__ double lds[];
     for (int t = blockIdx.x; t < num_tiles; t += gridDim.x) {
         const int tile = p.tile_ids ? p.tile_ids[t] : t;
@@ -470,7 +621,9 @@ __global__ __launch_bounds__(256) void k_energy_segments(const double* __restric
 // Projection: prepass (PSD test of the 9x9 material tangent A by LDL^T) and 9x9 Jacobi eigen-clamp of M.
 // ---------------------------------------------------------------------------------------------------------------
 struct ProjectParams {
-    const double* rec;
+    const double* Xr;
+    const int32_t* conn;
+    const double* theta_snap;
     const double* x;
     const double* theta;
     int num_tets;
@@ -488,54 +641,68 @@ __device__ __forceinline__ int sym_index(int r, int c) {
 
 __global__ __launch_bounds__(256) void k_project_prepass(ProjectParams p) {
     const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= p.num_tets)
-        return;
-    TetIn t;
-    load_tet(p.rec, p.theta, e, t);
-    double u[4][3];
-    load_u(p.x, t.ids, u);
-    Kin k;
-    kinematics(u, t.Di, k);
-    int32_t slot = -1;
-    if (kin_ok(k) && theta_ok(t.lam, t.mu)) {
-        const double pr = fma(t.lam, k.L, -t.mu);
-        // A(i+3j, m+3l) = mu d_im d_jl + lambda a_ij a_ml - p a_il a_mj, packed upper (45).
-        double A[45];
+    bool indefinite = false;
+    if (e < p.num_tets) {
+        int ids[4];
+        TetIn t;
+        load_tet(p.Xr, p.conn, p.theta_snap, p.x, p.theta, e, ids, t);
+        Geo geo;
+        rest_geometry(t, geo);
+        Kin k;
+        kinematics(t, geo.Di, k);
+        if (kin_ok(k) && theta_ok(t.lam, t.mu)) {
+            const double pr = fma(t.lam, k.L, -t.mu);
+            // A(i+3j, m+3l) = mu d_im d_jl + lambda a_ij a_ml - p a_il a_mj, packed upper (45).
+            double A[45];
 #pragma unroll
-        for (int r = 0; r < 9; ++r)
+            for (int r = 0; r < 9; ++r)
 #pragma unroll
-            for (int c = r; c < 9; ++c) {
-                const int i = r % 3, j = r / 3, m = c % 3, l = c / 3;
-                double val = fma(t.lam * k.a[3 * i + j], k.a[3 * m + l], -(pr * k.a[3 * i + l] * k.a[3 * m + j]));
-                if (i == m && j == l)
-                    val += t.mu;
-                A[sym_index(r, c)] = val;
+                for (int c = r; c < 9; ++c) {
+                    const int i = r % 3, j = r / 3, m = c % 3, l = c / 3;
+                    double val = fma(t.lam * k.a[3 * i + j], k.a[3 * m + l], -(pr * k.a[3 * i + l] * k.a[3 * m + j]));
+                    if (i == m && j == l)
+                        val += t.mu;
+                    A[sym_index(r, c)] = val;
+                }
+            double scale = 0.0;
+#pragma unroll
+            for (int r = 0; r < 9; ++r)
+                scale = fmax(scale, fabs(A[sym_index(r, r)]));
+            // In-place LDL^T; positive definite iff every pivot is positive.
+            bool pd = true;
+#pragma unroll
+            for (int j = 0; j < 9; ++j) {
+                const double d = A[sym_index(j, j)];
+                pd = pd && (d > 1e-13 * scale);
+                const double inv = 1.0 / d;
+#pragma unroll
+                for (int i = j + 1; i < 9; ++i) {
+                    const double lij = A[sym_index(j, i)] * inv;
+#pragma unroll
+                    for (int c = i; c < 9; ++c)
+                        A[sym_index(i, c)] = fma(-lij, A[sym_index(j, c)], A[sym_index(i, c)]);
+                }
             }
-        double scale = 0.0;
-#pragma unroll
-        for (int r = 0; r < 9; ++r)
-            scale = fmax(scale, fabs(A[sym_index(r, r)]));
-        // In-place LDL^T; positive definite iff every pivot is positive.
-        bool pd = true;
-#pragma unroll
-        for (int j = 0; j < 9; ++j) {
-            const double d = A[sym_index(j, j)];
-            pd = pd && (d > 1e-13 * scale);
-            const double inv = 1.0 / d;
-#pragma unroll
-            for (int i = j + 1; i < 9; ++i) {
-                const double lij = A[sym_index(j, i)] * inv;
-#pragma unroll
-                for (int c = i; c < 9; ++c)
-                    A[sym_index(i, c)] -= lij * A[sym_index(j, c)];
-            }
+            indefinite = !pd;
         }
-        if (!pd) {
-            slot = static_cast<int32_t>(atomicAdd(p.counter, 1u));
+    }
+    // One atomic per wave: the wave's indefinite lanes take consecutive slots in lane order.
+    const unsigned long long mask = __ballot(indefinite);
+    int32_t slot = -1;
+    if (mask) {
+        const int lane = static_cast<int>(__lane_id());
+        const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+        unsigned base = 0;
+        if (lane == leader)
+            base = atomicAdd(p.counter, static_cast<unsigned>(__popcll(mask)));
+        base = __shfl(base, leader);
+        if (indefinite) {
+            slot = static_cast<int32_t>(base + __popcll(mask & ((1ull << lane) - 1ull)));
             p.proj_list[slot] = e;
         }
     }
-    p.proj_slot[e] = slot;
+    if (e < p.num_tets)
+        p.proj_slot[e] = slot;
 }
 
 // One thread per indefinite element (grid-stride over the compacted list).
@@ -543,14 +710,15 @@ __global__ __launch_bounds__(64) void k_project_jacobi(ProjectParams p) {
     const unsigned count = *p.counter;
     for (unsigned idx = blockIdx.x * 64 + threadIdx.x; idx < count; idx += gridDim.x * 64) {
         const int e = p.proj_list[idx];
+        int ids[4];
         TetIn t;
-        load_tet(p.rec, p.theta, e, t);
-        double u[4][3];
-        load_u(p.x, t.ids, u);
+        load_tet(p.Xr, p.conn, p.theta_snap, p.x, p.theta, e, ids, t);
+        Geo geo;
+        rest_geometry(t, geo);
         Kin k;
-        kinematics(u, t.Di, k);
+        kinematics(t, geo.Di, k);
         double G[4][3];
-        shape_gradients(t.Di, G);
+        shape_gradients(geo.Di, G);
         const double pr = fma(t.lam, k.L, -t.mu);
         // c_m = sum_a P(a,m) g_a, r_m = F^-T c_m.
         double cm[3][3], rm[3][3];
@@ -575,7 +743,7 @@ __global__ __launch_bounds__(64) void k_project_jacobi(ProjectParams p) {
                 double val = fma(t.lam * rm[m][i], rm[n][kk], -(pr * rm[n][i] * rm[m][kk]));
                 if (i == kk)
                     val = fma(t.mu, dot3(cm[m][0], cm[m][1], cm[m][2], cm[n][0], cm[n][1], cm[n][2]), val);
-                M[sym_index(r, c)] = t.w * val;
+                M[sym_index(r, c)] = geo.w * val;
             }
         double V[9][9];
 #pragma unroll
@@ -595,7 +763,7 @@ __global__ __launch_bounds__(64) void k_project_jacobi(ProjectParams p) {
 #pragma unroll
                 for (int c = r + 1; c < 9; ++c)
                     off = fmax(off, fabs(M[sym_index(r, c)]));
-            if (off <= 1e-17 * scale || off == 0.0) {
+            if (off <= 1e-15 * scale || off == 0.0) {
                 converged = true;
                 break;
             }
@@ -604,7 +772,7 @@ __global__ __launch_bounds__(64) void k_project_jacobi(ProjectParams p) {
 #pragma unroll
                 for (int qq = pp + 1; qq < 9; ++qq) {
                     const double apq = M[sym_index(pp, qq)];
-                    if (apq != 0.0) {
+                    if (fabs(apq) > 1e-18 * scale) {
                         const double app = M[sym_index(pp, pp)], aqq = M[sym_index(qq, qq)];
                         const double tau = (aqq - app) / (2.0 * apq);
                         const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
@@ -636,7 +804,7 @@ __global__ __launch_bounds__(64) void k_project_jacobi(ProjectParams p) {
 #pragma unroll
                 for (int c = r + 1; c < 9; ++c)
                     off = fmax(off, fabs(M[sym_index(r, c)]));
-            converged = off <= 1e-15 * scale;
+            converged = off <= 1e-13 * scale;
         }
         double lam[9];
 #pragma unroll
@@ -664,7 +832,9 @@ __global__ __launch_bounds__(64) void k_project_jacobi(ProjectParams p) {
 // Differentiation kernels.
 // ---------------------------------------------------------------------------------------------------------------
 struct ElemParams {
-    const double* rec;
+    const double* Xr;
+    const int32_t* conn;
+    const double* theta_snap;
     const double* x;
     const double* theta;
     const double* w;      // adjoint vector (num_dofs) for the VJP
@@ -678,12 +848,13 @@ __global__ __launch_bounds__(256) void k_dE_dtheta(ElemParams p) {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= p.num_tets)
         return;
+    int ids[4];
     TetIn t;
-    load_tet(p.rec, p.theta, e, t);
-    double u[4][3];
-    load_u(p.x, t.ids, u);
+    load_tet(p.Xr, p.conn, p.theta_snap, p.x, p.theta, e, ids, t);
+    Geo geo;
+    rest_geometry(t, geo);
     Kin k;
-    kinematics(u, t.Di, k);
+    kinematics(t, geo.Di, k);
     unsigned flags = 0;
     if (!kin_ok(k))
         flags |= kStatusDetF;
@@ -694,8 +865,8 @@ __global__ __launch_bounds__(256) void k_dE_dtheta(ElemParams p) {
     for (int q = 0; q < 9; ++q)
         sq = fma(k.F[q], k.F[q], sq);
     double2 r;
-    r.x = t.w * (0.5 * k.L * k.L);
-    r.y = t.w * (0.5 * (sq - 3.0) - k.L);
+    r.x = geo.w * (0.5 * k.L * k.L);
+    r.y = geo.w * (0.5 * (sq - 3.0) - k.L);
     if (!(isfinite(r.x) && isfinite(r.y)))
         flags |= kStatusNonFiniteOut;
     reinterpret_cast<double2*>(p.out)[e] = r;
@@ -709,23 +880,24 @@ __global__ __launch_bounds__(256) void k_force_vjp_theta(ElemParams p) {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= p.num_tets)
         return;
+    int ids[4];
     TetIn t;
-    load_tet(p.rec, p.theta, e, t);
-    double u[4][3];
-    load_u(p.x, t.ids, u);
+    load_tet(p.Xr, p.conn, p.theta_snap, p.x, p.theta, e, ids, t);
+    Geo geo;
+    rest_geometry(t, geo);
     Kin k;
-    kinematics(u, t.Di, k);
+    kinematics(t, geo.Di, k);
     unsigned flags = 0;
     if (!kin_ok(k))
         flags |= kStatusDetF;
     if (!theta_ok(t.lam, t.mu))
         flags |= kStatusBadTheta;
     double G[4][3];
-    shape_gradients(t.Di, G);
+    shape_gradients(geo.Di, G);
     double sl = 0.0, sm = 0.0;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-        const double* wb = p.w + 3 * static_cast<size_t>(t.ids[b]);
+        const double* wb = p.w + 3 * static_cast<size_t>(ids[b]);
         const double w0 = wb[0], w1 = wb[1], w2 = wb[2];
         double q[3], Fg[3];
 #pragma unroll
@@ -739,8 +911,8 @@ __global__ __launch_bounds__(256) void k_force_vjp_theta(ElemParams p) {
         sm += fw - qw;
     }
     double2 r;
-    r.x = t.w * k.L * sl;
-    r.y = t.w * sm;
+    r.x = geo.w * k.L * sl;
+    r.y = geo.w * sm;
     if (!(isfinite(r.x) && isfinite(r.y)))
         flags |= kStatusNonFiniteOut;
     reinterpret_cast<double2*>(p.out)[e] = r;
@@ -750,7 +922,9 @@ __global__ __launch_bounds__(256) void k_force_vjp_theta(ElemParams p) {
 
 // Vertex-gather products: MODE 0 = Exact H v, MODE 1 = sum_e (d g_e / d theta_e) dtheta_e.
 struct GatherParams {
-    const double* rec;
+    const double* Xr;
+    const int32_t* conn;
+    const double* theta_snap;
     const int32_t* inc_ptr;
     const int32_t* inc_tet; // stride (bytes) given by inc_stride
     int inc_stride;
@@ -772,19 +946,20 @@ template <int MODE> __global__ __launch_bounds__(256) void k_vertex_gather(Gathe
     for (int kk = kb; kk < ke; ++kk) {
         const int e = *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(p.inc_tet) +
                                                          static_cast<size_t>(kk) * p.inc_stride);
+        int ids[4];
         TetIn t;
-        load_tet(p.rec, p.theta, e, t);
-        double u[4][3];
-        load_u(p.x, t.ids, u);
-        const int a = (t.ids[0] == v) ? 0 : (t.ids[1] == v) ? 1 : (t.ids[2] == v) ? 2 : 3;
+        load_tet(p.Xr, p.conn, p.theta_snap, p.x, p.theta, e, ids, t);
+        Geo geo;
+        rest_geometry(t, geo);
+        const int a = (ids[0] == v) ? 0 : (ids[1] == v) ? 1 : (ids[2] == v) ? 2 : 3;
         Kin k;
-        kinematics(u, t.Di, k);
+        kinematics(t, geo.Di, k);
         if (!kin_ok(k))
             flags |= kStatusDetF;
         if (!theta_ok(t.lam, t.mu))
             flags |= kStatusBadTheta;
         double G[4][3];
-        shape_gradients(t.Di, G);
+        shape_gradients(geo.Di, G);
         double ga[3];
         select_row(G, a, ga);
         double qa[3];
@@ -798,7 +973,7 @@ template <int MODE> __global__ __launch_bounds__(256) void k_vertex_gather(Gathe
             double sqv = 0.0;
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const double* vb = p.v + 3 * static_cast<size_t>(t.ids[b]);
+                const double* vb = p.v + 3 * static_cast<size_t>(ids[b]);
                 const double vb0 = vb[0], vb1 = vb[1], vb2 = vb[2];
                 double qb[3];
 #pragma unroll
@@ -814,9 +989,9 @@ template <int MODE> __global__ __launch_bounds__(256) void k_vertex_gather(Gathe
                 sq[1] = fma(qb[1], qav, sq[1]);
                 sq[2] = fma(qb[2], qav, sq[2]);
             }
-            y0 = fma(t.w, fma(t.mu, sv[0], fma(t.lam * qa[0], sqv, -(pr * sq[0]))), y0);
-            y1 = fma(t.w, fma(t.mu, sv[1], fma(t.lam * qa[1], sqv, -(pr * sq[1]))), y1);
-            y2 = fma(t.w, fma(t.mu, sv[2], fma(t.lam * qa[2], sqv, -(pr * sq[2]))), y2);
+            y0 = fma(geo.w, fma(t.mu, sv[0], fma(t.lam * qa[0], sqv, -(pr * sq[0]))), y0);
+            y1 = fma(geo.w, fma(t.mu, sv[1], fma(t.lam * qa[1], sqv, -(pr * sq[1]))), y1);
+            y2 = fma(geo.w, fma(t.mu, sv[2], fma(t.lam * qa[2], sqv, -(pr * sq[2]))), y2);
         } else {
             const double2 dt = reinterpret_cast<const double2*>(p.v)[e];
             double Fg[3];
@@ -824,9 +999,9 @@ template <int MODE> __global__ __launch_bounds__(256) void k_vertex_gather(Gathe
             for (int i = 0; i < 3; ++i)
                 Fg[i] = dot3(k.F[3 * i], k.F[3 * i + 1], k.F[3 * i + 2], ga[0], ga[1], ga[2]);
             const double dl = dt.x * k.L;
-            y0 = fma(t.w, fma(dl, qa[0], dt.y * (Fg[0] - qa[0])), y0);
-            y1 = fma(t.w, fma(dl, qa[1], dt.y * (Fg[1] - qa[1])), y1);
-            y2 = fma(t.w, fma(dl, qa[2], dt.y * (Fg[2] - qa[2])), y2);
+            y0 = fma(geo.w, fma(dl, qa[0], dt.y * (Fg[0] - qa[0])), y0);
+            y1 = fma(geo.w, fma(dl, qa[1], dt.y * (Fg[1] - qa[1])), y1);
+            y2 = fma(geo.w, fma(dl, qa[2], dt.y * (Fg[2] - qa[2])), y2);
         }
     }
     double* o = p.out + 3 * static_cast<size_t>(v);
@@ -853,65 +1028,78 @@ template <class Kernel> void allow_large_lds(Kernel kern, int waves) {
         fail(kDevice, std::string("hipFuncSetAttribute: ") + hipGetErrorString(err));
 }
 
-template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE>
-void launch_assemble_t(const AssembleParams& p, int num_tiles, int lds_doubles, bool global_acc,
-                       int persistent_grid, hipStream_t stream) {
+template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GACC>
+void launch_assemble_g(const AssembleParams& p, int num_tiles, size_t bytes, int persistent_grid,
+                       hipStream_t stream) {
     const dim3 block(kWave * WAVES);
-    const size_t bytes = global_acc ? 0 : static_cast<size_t>(lds_doubles) * sizeof(double);
     if (persistent_grid > 0) {
-        const dim3 grid(std::min(persistent_grid, num_tiles));
-        if (global_acc) {
-            hipLaunchKernelGGL((k_assemble_persistent<LPV, WAVES, WANT_G, KIND, WIDE, true>), grid, block, 0,
-                               stream, p, num_tiles);
-        } else {
-            auto kern = k_assemble_persistent<LPV, WAVES, WANT_G, KIND, WIDE, false>;
-            static bool configured = false;
-            if (!configured) {
-                allow_large_lds(kern, WAVES);
-                configured = true;
-            }
-            hipLaunchKernelGGL(kern, grid, block, bytes, stream, p, num_tiles);
+        auto kern = k_assemble_persistent<LPV, WAVES, WANT_G, KIND, WIDE, GACC>;
+        static bool configured = false;
+        if (!configured) {
+            allow_large_lds(kern, WAVES);
+            configured = true;
         }
+        hipLaunchKernelGGL(kern, dim3(std::min(persistent_grid, num_tiles)), block, bytes, stream, p, num_tiles);
     } else {
-        const dim3 grid(num_tiles);
-        if (global_acc) {
-            hipLaunchKernelGGL((k_assemble<LPV, WAVES, WANT_G, KIND, WIDE, true>), grid, block, 0, stream, p);
-        } else {
-            auto kern = k_assemble<LPV, WAVES, WANT_G, KIND, WIDE, false>;
-            static bool configured = false;
-            if (!configured) {
-                allow_large_lds(kern, WAVES);
-                configured = true;
-            }
-            hipLaunchKernelGGL(kern, grid, block, bytes, stream, p);
+        auto kern = k_assemble<LPV, WAVES, WANT_G, KIND, WIDE, GACC>;
+        static bool configured = false;
+        if (!configured) {
+            allow_large_lds(kern, WAVES);
+            configured = true;
         }
+        hipLaunchKernelGGL(kern, dim3(num_tiles), block, bytes, stream, p);
     }
     check_launch("k_assemble");
 }
 
-template <int LPV, int WAVES>
-void dispatch_kind(const AssembleParams& p, int want_g, int kind, bool wide, int num_tiles, int lds, bool gacc,
+// Wide incidence records (vertex degree above 255) are only instantiated for LPV = 4, WAVES = 1.
+template <int LPV, int WAVES, bool WIDE>
+void dispatch_kind(const AssembleParams& p, int want_g, int kind, int num_tiles, size_t bytes, bool gacc,
                    int persistent_grid, hipStream_t s) {
-#define LPG_CASE(G, K, W)                                                                                            \
-    if (!!want_g == G && kind == K && wide == W)                                                                      \
-        return launch_assemble_t<LPV, WAVES, G, K, W>(p, num_tiles, lds, gacc, persistent_grid, s);
-    LPG_CASE(false, -1, false)
-    LPG_CASE(true, -1, false)
-    LPG_CASE(false, 0, false)
-    LPG_CASE(true, 0, false)
-    LPG_CASE(false, 1, false)
-    LPG_CASE(true, 1, false)
-    LPG_CASE(false, -1, true)
-    LPG_CASE(true, -1, true)
-    LPG_CASE(false, 0, true)
-    LPG_CASE(true, 0, true)
-    LPG_CASE(false, 1, true)
-    LPG_CASE(true, 1, true)
+#define LPG_CASE(G, K)                                                                                               \
+    if (!!want_g == G && kind == K) {                                                                                 \
+        if (gacc)                                                                                                     \
+            return launch_assemble_g<LPV, WAVES, G, K, WIDE, true>(p, num_tiles, bytes, persistent_grid, s);         \
+        return launch_assemble_g<LPV, WAVES, G, K, WIDE, false>(p, num_tiles, bytes, persistent_grid, s);            \
+    }
+    LPG_CASE(false, -1)
+    LPG_CASE(true, -1)
+    LPG_CASE(false, 0)
+    LPG_CASE(true, 0)
+    LPG_CASE(false, 1)
+    LPG_CASE(true, 1)
 #undef LPG_CASE
     fail(kLogic, "k_assemble: unsupported configuration");
 }
 
 } // namespace
+
+// Compiled launch shapes (lanes per vertex, waves per workgroup).  Meshes with a vertex of more than 255 neighbours
+// (16-bit column positions) use (4, 1) only.
+#define LPG_SHAPES(X) X(1, 1) X(2, 1) X(4, 1) X(5, 1) X(6, 1) X(8, 1) X(4, 2) X(6, 2) X(8, 2)
+
+bool launch_shape_supported(int lanes_per_vertex, int waves, bool wide) {
+    if (wide)
+        return lanes_per_vertex == 4 && waves == 1;
+#define LPG_IS(L, W)                                                                                                 \
+    if (lanes_per_vertex == L && waves == W)                                                                          \
+        return true;
+    LPG_SHAPES(LPG_IS)
+#undef LPG_IS
+    return false;
+}
+
+size_t assemble_lds_bytes(const LaunchConfig& lc, const TileCaps& caps, bool global_acc, int kind) {
+    const size_t verts = static_cast<size_t>(lc.waves * (kWave / lc.lanes_per_vertex));
+    size_t doubles = 4 * verts; // gradient + energy slots
+    if (!global_acc && kind >= 0)
+        doubles += static_cast<size_t>(caps.segment);
+    return doubles * sizeof(double);
+}
+
+size_t assemble_lds_limit(const LaunchConfig& lc) {
+    return 160 * 1024 - static_cast<size_t>(kWave) * lc.waves * 8;
+}
 
 void launch_clear(unsigned* status, unsigned* counter, hipStream_t stream) {
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, stream, status, counter);
@@ -919,14 +1107,17 @@ void launch_clear(unsigned* status, unsigned* counter, hipStream_t stream) {
 }
 
 void launch_assemble(const DeviceMesh& m, const EvalArgs& a, const LaunchConfig& lc, const int32_t* tile_vbeg,
-                     const int32_t* tile_ids, int num_tiles, int lds_doubles, bool global_acc, int persistent_grid,
-                     hipStream_t stream) {
+                     const int32_t* tile_ids, int num_tiles, const TileCaps& caps, bool global_acc,
+                     int persistent_grid, hipStream_t stream) {
     if (num_tiles <= 0)
         return;
     AssembleParams p{};
-    p.rec = m.rec;
+    p.Xr = m.Xr;
+    p.theta_snap = m.theta;
     p.inc_ptr = m.inc_ptr;
     p.inc = m.inc;
+    p.nbr_ptr = m.nbr_ptr;
+    p.nbr = m.nbr;
     p.row_ptr = m.row_ptr;
     p.tile_vbeg = tile_vbeg;
     p.tile_ids = tile_ids;
@@ -938,19 +1129,19 @@ void launch_assemble(const DeviceMesh& m, const EvalArgs& a, const LaunchConfig&
     p.status = a.status;
     p.proj_slot = a.proj_slot;
     p.proj_M = a.proj_M;
+    const size_t bytes = assemble_lds_bytes(lc, caps, global_acc, a.kind);
+    if (bytes > assemble_lds_limit(lc))
+        fail(kLogic, "launch_assemble: staged tile data exceeds the LDS of a workgroup");
+    if (m.wide) {
+        if (lc.lanes_per_vertex == 4 && lc.waves == 1)
+            return dispatch_kind<4, 1, true>(p, a.want_g, a.kind, num_tiles, bytes, global_acc, persistent_grid,
+                                             stream);
+        fail(kInvalid, "launch_assemble: meshes with vertex degree above 255 use 4 lanes per vertex and one wave");
+    }
 #define LPG_CFG(L, W)                                                                                                \
     if (lc.lanes_per_vertex == L && lc.waves == W)                                                                    \
-        return dispatch_kind<L, W>(p, a.want_g, a.kind, m.wide, num_tiles, lds_doubles, global_acc,           \
-                                   persistent_grid, stream);
-    LPG_CFG(1, 1)
-    LPG_CFG(2, 1)
-    LPG_CFG(4, 1)
-    LPG_CFG(1, 2)
-    LPG_CFG(2, 2)
-    LPG_CFG(4, 2)
-    LPG_CFG(1, 4)
-    LPG_CFG(2, 4)
-    LPG_CFG(4, 4)
+        return dispatch_kind<L, W, false>(p, a.want_g, a.kind, num_tiles, bytes, global_acc, persistent_grid, stream);
+    LPG_SHAPES(LPG_CFG)
 #undef LPG_CFG
     fail(kInvalid, "launch_assemble: unsupported launch configuration");
 }
@@ -972,7 +1163,7 @@ void launch_energy_reduce(const double* vertex_E, const int32_t* chunk_begin, in
 void launch_project_prepass(const DeviceMesh& m, const EvalArgs& a, hipStream_t stream) {
     if (m.num_tets == 0)
         return;
-    ProjectParams p{m.rec, a.x, a.theta, m.num_tets, a.status, a.counter, a.proj_slot, a.proj_list, a.proj_M};
+    ProjectParams p{m.Xr, m.conn, m.theta, a.x, a.theta, m.num_tets, a.status, a.counter, a.proj_slot, a.proj_list, a.proj_M};
     hipLaunchKernelGGL(k_project_prepass, dim3((m.num_tets + 255) / 256), dim3(256), 0, stream, p);
     check_launch("k_project_prepass");
 }
@@ -980,7 +1171,7 @@ void launch_project_prepass(const DeviceMesh& m, const EvalArgs& a, hipStream_t 
 void launch_project_jacobi(const DeviceMesh& m, const EvalArgs& a, hipStream_t stream) {
     if (m.num_tets == 0)
         return;
-    ProjectParams p{m.rec, a.x, a.theta, m.num_tets, a.status, a.counter, a.proj_slot, a.proj_list, a.proj_M};
+    ProjectParams p{m.Xr, m.conn, m.theta, a.x, a.theta, m.num_tets, a.status, a.counter, a.proj_slot, a.proj_list, a.proj_M};
     const int blocks = std::min(4096, (m.num_tets + 63) / 64);
     hipLaunchKernelGGL(k_project_jacobi, dim3(blocks), dim3(64), 0, stream, p);
     check_launch("k_project_jacobi");
@@ -990,7 +1181,7 @@ void launch_dE_dtheta(const DeviceMesh& m, const double* x, const double* theta,
                       hipStream_t stream) {
     if (m.num_tets == 0)
         return;
-    ElemParams p{m.rec, x, theta, nullptr, out, status, m.num_tets};
+    ElemParams p{m.Xr, m.conn, m.theta, x, theta, nullptr, out, status, m.num_tets};
     hipLaunchKernelGGL(k_dE_dtheta, dim3((m.num_tets + 255) / 256), dim3(256), 0, stream, p);
     check_launch("k_dE_dtheta");
 }
@@ -999,7 +1190,7 @@ void launch_force_vjp_theta(const DeviceMesh& m, const double* x, const double* 
                             double* out, unsigned* status, hipStream_t stream) {
     if (m.num_tets == 0)
         return;
-    ElemParams p{m.rec, x, theta, w, out, status, m.num_tets};
+    ElemParams p{m.Xr, m.conn, m.theta, x, theta, w, out, status, m.num_tets};
     hipLaunchKernelGGL(k_force_vjp_theta, dim3((m.num_tets + 255) / 256), dim3(256), 0, stream, p);
     check_launch("k_force_vjp_theta");
 }
@@ -1008,7 +1199,9 @@ static void launch_gather(int mode, const DeviceMesh& m, const double* x, const 
                           double* out, unsigned* status, hipStream_t stream) {
     if (m.num_vertices == 0)
         return;
-    GatherParams p{m.rec,
+    GatherParams p{m.Xr,
+                   m.conn,
+                   m.theta,
                    m.inc_ptr,
                    static_cast<const int32_t*>(m.inc),
                    m.wide ? static_cast<int>(sizeof(IncWide)) : static_cast<int>(sizeof(IncNarrow)),

@@ -1,4 +1,4 @@
-// Host-side construction: validation, per-tet records, the sparsity pattern, vertex incidence lists and tiles.
+// Host-side construction: validation, the device arrays, the sparsity pattern, vertex incidence lists and tiles.
 //
 // The pattern is built at vertex granularity.  The reference forms every (row, col) pair of every element's 12
 // DOFs and deduplicates them with setFromTriplets (src/core/dof/energy_assembler.cpp:146-154); because a Tet P1
@@ -13,6 +13,9 @@
 #include <cmath>
 #include <limits>
 #include <thread>
+
+// Explicit fma only (the geometry check mirrors the kernels bit for bit).
+#pragma clang fp contract(off)
 
 namespace lpg {
 namespace {
@@ -78,9 +81,10 @@ void build_mesh_setup(MeshSetup& s, int32_t nv, int32_t nt, const int32_t* tets,
     s.num_tets = nt;
     s.num_dofs = 3 * nv;
 
-    // Per-tet records: VolumetricElement3D (volumetric_element.hpp:23-39) and the deformation-gradient sampling
-    // (deformation_gradient_sampling_response.hpp:24-41): Dm = [X1-X0 | X2-X0 | X3-X0], w = det(Dm)/6, Dm^-1.
-    s.rec.assign(static_cast<size_t>(nt) * kRecDoubles, 0.0);
+    // Rest geometry checks of VolumetricElement3D (volumetric_element.hpp:23-39) and the deformation-gradient
+    // sampling (deformation_gradient_sampling_response.hpp:24-41): Dm = [X1-X0 | X2-X0 | X3-X0] must have a finite,
+    // positive determinant, w = det(Dm)/6 and Dm^-1 must be finite.  The kernels recompute Dm^-1 and w from the rest
+    // positions, so nothing per tet is stored beyond connectivity and theta.
     std::atomic<int64_t> bad_element{std::numeric_limits<int64_t>::max()};
     std::atomic<int> bad_kind{0};
     parallel_for(nt, nthreads, [&](int64_t e) {
@@ -100,31 +104,27 @@ void build_mesh_setup(MeshSetup& s, int32_t nv, int32_t nt, const int32_t* tets,
             note(1);
             return;
         }
+        // Same operation sequence as the kernels' rest_geometry (explicit fma, 1/det then products), so the
+        // host and the device agree bitwise on det, w and Dm^-1.
         double C[9];
         for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j)
-                C[3 * i + j] = Dm[3 * ((i + 1) % 3) + (j + 1) % 3] * Dm[3 * ((i + 2) % 3) + (j + 2) % 3] -
-                               Dm[3 * ((i + 1) % 3) + (j + 2) % 3] * Dm[3 * ((i + 2) % 3) + (j + 1) % 3];
-        const double det = Dm[0] * C[0] + Dm[1] * C[1] + Dm[2] * C[2];
-        const double w = (1.0 / 6) * det;
+            for (int j = 0; j < 3; ++j) {
+                const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+                C[3 * i + j] = std::fma(Dm[3 * i1 + j1], Dm[3 * i2 + j2], -(Dm[3 * i1 + j2] * Dm[3 * i2 + j1]));
+            }
+        const double det = std::fma(Dm[2], C[2], std::fma(Dm[1], C[1], Dm[0] * C[0]));
+        const double w = det * (1.0 / 6.0);
         if (!std::isfinite(det) || det <= 0 || !std::isfinite(w)) {
             note(2);
             return;
         }
-        double* R = &s.rec[kRecDoubles * e];
+        const double inv = 1.0 / det;
+        double Dinv[9];
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j)
-                R[3 * i + j] = C[3 * j + i] / det; // Dm^-1(i, j) = C(j, i) / det
-        if (!finite9(R)) {
+                Dinv[3 * i + j] = C[3 * j + i] * inv;
+        if (!finite9(Dinv))
             note(3);
-            return;
-        }
-        R[9] = w;
-        R[10] = theta[2 * e];
-        R[11] = theta[2 * e + 1];
-        int32_t* ids = reinterpret_cast<int32_t*>(R + 12);
-        for (int k = 0; k < 4; ++k)
-            ids[k] = c[k];
     });
     if (bad_element.load() != std::numeric_limits<int64_t>::max()) {
         const int kind = bad_kind.load();
@@ -132,6 +132,12 @@ void build_mesh_setup(MeshSetup& s, int32_t nv, int32_t nt, const int32_t* tets,
                        : kind == 2 ? "VolumetricElement3D: invalid reference determinant or volume weight"
                                    : "Deformation gradient sampling: non-finite physical gradient");
     }
+    s.conn.assign(tets, tets + 4 * static_cast<int64_t>(nt));
+    s.theta.assign(theta, theta + 2 * static_cast<int64_t>(nt));
+    s.Xr.assign(4 * static_cast<size_t>(nv), 0.0);
+    for (int64_t v = 0; v < nv; ++v)
+        for (int c = 0; c < 3; ++c)
+            s.Xr[4 * v + c] = rest[3 * v + c];
     // NeoHookean checks theta when it evaluates (neo_hookean.cpp:15-16); remember the outcome for eval.
     s.theta_valid = true;
     for (int64_t e = 0; e < nt; ++e) {
@@ -225,9 +231,8 @@ void build_mesh_setup(MeshSetup& s, int32_t nv, int32_t nt, const int32_t* tets,
 void build_slot_map(const MeshSetup& s, int32_t* offsets) {
     // offsets[144 e + i + 12 j] = CSC position of (row idx[i], col idx[j]) = CSR position of (row idx[j], col idx[i])
     // (the pattern is symmetric, and CSC column c is CSR row c).
-    const double* rec = s.rec.data();
     parallel_for(s.num_tets, 8, [&](int64_t e) {
-        const int32_t* ids = reinterpret_cast<const int32_t*>(rec + kRecDoubles * e + 12);
+        const int32_t* ids = s.conn.data() + 4 * e;
         for (int j = 0; j < 12; ++j) {
             const int32_t vj = ids[j / 3], cj = j % 3;
             const int32_t* nb_begin = s.nbr.data() + s.nbr_ptr[vj];
@@ -241,33 +246,30 @@ void build_slot_map(const MeshSetup& s, int32_t* offsets) {
     });
 }
 
-void build_tiles(const MeshSetup& s, const TileConfig& cfg, std::vector<int32_t>& tile_vbeg, int32_t& max_segment,
+void build_tiles(const MeshSetup& s, const TileConfig& cfg, std::vector<int32_t>& tile_vbeg, TileCaps& caps,
                  std::vector<int32_t>* oversized, const std::vector<int32_t>* breaks) {
     tile_vbeg.clear();
-    max_segment = 0;
+    caps = TileCaps{};
     if (oversized)
         oversized->clear();
     const int32_t nv = s.num_vertices;
+    auto segment = [&](int32_t vb, int32_t ve) { return int64_t{9} * (s.nbr_ptr[ve] - s.nbr_ptr[vb]); };
     int32_t v = 0;
     size_t next_break = 0;
     tile_vbeg.push_back(0);
     while (v < nv) {
         const int32_t start = v;
-        int64_t seg = 0;
         while (breaks && next_break < breaks->size() && (*breaks)[next_break] <= start)
             ++next_break;
         const int32_t limit = (breaks && next_break < breaks->size()) ? std::min(nv, (*breaks)[next_break]) : nv;
-        while (v < limit && v - start < cfg.vertices_per_tile) {
-            const int64_t len = 9 * static_cast<int64_t>(s.nbr_ptr[v + 1] - s.nbr_ptr[v]);
-            if (v > start && seg + len > cfg.lds_budget_doubles)
-                break;
-            seg += len;
+        ++v; // a tile holds at least one vertex
+        while (v < limit && v - start < cfg.vertices_per_tile && segment(start, v + 1) <= cfg.lds_budget_doubles)
             ++v;
-        }
+        const int64_t seg = segment(start, v);
         if (seg > cfg.lds_budget_doubles && oversized)
             oversized->push_back(static_cast<int32_t>(tile_vbeg.size() - 1));
         else
-            max_segment = std::max<int32_t>(max_segment, static_cast<int32_t>(seg));
+            caps.segment = std::max<int32_t>(caps.segment, static_cast<int32_t>(seg));
         tile_vbeg.push_back(v);
     }
 }
