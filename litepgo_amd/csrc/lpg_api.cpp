@@ -66,6 +66,7 @@ lpg::DeviceMesh lpg_tet_energy::mesh() const {
     m.inc = d_inc.ptr;
     m.nbr_ptr = d_nbr_ptr.as<int32_t>();
     m.nbr = d_nbr.as<int32_t>();
+    m.self_pos = d_self_pos.as<int32_t>();
     m.row_ptr = d_row_ptr.as<int32_t>();
     m.num_vertices = setup.num_vertices;
     m.num_tets = setup.num_tets;
@@ -74,8 +75,8 @@ lpg::DeviceMesh lpg_tet_energy::mesh() const {
 }
 
 lpg_tet_energy::~lpg_tet_energy() {
-    for (lpg::DeviceBuffer* b : {&d_Xr, &d_conn, &d_theta, &d_inc_ptr, &d_inc, &d_nbr_ptr, &d_nbr, &d_row_ptr, &d_scalars, &d_vertex_E, &d_chunk_begin,
-                                 &d_chunk_sum, &d_seg_chunk, &d_proj_slot, &d_proj_list, &d_proj_M, &d_x, &d_g, &d_H,
+    for (lpg::DeviceBuffer* b : {&d_Xr, &d_conn, &d_theta, &d_inc_ptr, &d_inc, &d_nbr_ptr, &d_nbr, &d_self_pos, &d_row_ptr, &d_scalars, &d_vertex_E, &d_chunk_begin,
+                                 &d_chunk_sum, &d_seg_chunk, &d_proj_slot, &d_proj_list, &d_proj_H, &d_x, &d_g, &d_H,
                                  &plan.d_tile_vbeg, &plan.d_normal_ids, &plan.d_oversized_ids})
         b->release();
     if (pinned)
@@ -134,26 +135,28 @@ void ensure_device(lpg_tet_energy* h) {
     upload(h->d_nbr_ptr, s.nbr_ptr.data(), s.nbr_ptr.size() * sizeof(int32_t));
     upload(h->d_nbr, s.nbr.data(), s.nbr.size() * sizeof(int32_t));
     upload(h->d_row_ptr, s.row_ptr.data(), s.row_ptr.size() * sizeof(int32_t));
+    upload(h->d_self_pos, s.self_pos.data(), s.self_pos.size() * sizeof(int32_t));
+    // Device incidence records (IncNarrow / IncWide): slot a of v in the tet, and the column positions of the tet's
+    // vertices at slots a^1, a^2, a^3.
     const size_t ninc = s.inc_tet.size();
-    if (!s.wide) {
-        std::vector<IncNarrow> inc(ninc);
-        for (size_t k = 0; k < ninc; ++k) {
-            uint32_t packed = 0;
-            for (int b = 0; b < 4; ++b)
-                packed |= static_cast<uint32_t>(s.inc_colpos[4 * k + b]) << (8 * b);
-            inc[k] = {s.inc_tet[k], packed};
+    std::vector<uint32_t> words(ninc * (s.wide ? 4 : 2));
+    for (int32_t v = 0; v < s.num_vertices; ++v)
+        for (int32_t k = s.inc_ptr[v]; k < s.inc_ptr[v + 1]; ++k) {
+            const int32_t* ids = s.conn.data() + 4 * static_cast<int64_t>(s.inc_tet[k]);
+            const uint32_t a = ids[0] == v ? 0u : ids[1] == v ? 1u : ids[2] == v ? 2u : 3u;
+            const uint16_t* cp = s.inc_colpos.data() + 4 * static_cast<int64_t>(k);
+            const uint32_t c1 = cp[a ^ 1u], c2 = cp[a ^ 2u], c3 = cp[a ^ 3u];
+            if (!s.wide) {
+                words[2 * k] = static_cast<uint32_t>(s.inc_tet[k]);
+                words[2 * k + 1] = c1 | (c2 << 8) | (c3 << 16) | (a << 24);
+            } else {
+                words[4 * k] = static_cast<uint32_t>(s.inc_tet[k]);
+                words[4 * k + 1] = c1 | (c2 << 16);
+                words[4 * k + 2] = c3 | (a << 16);
+                words[4 * k + 3] = 0;
+            }
         }
-        upload(h->d_inc, inc.data(), inc.size() * sizeof(IncNarrow));
-    } else {
-        std::vector<IncWide> inc(ninc);
-        for (size_t k = 0; k < ninc; ++k) {
-            inc[k].tet = s.inc_tet[k];
-            inc[k].colpos01 = s.inc_colpos[4 * k] | (static_cast<uint32_t>(s.inc_colpos[4 * k + 1]) << 16);
-            inc[k].colpos23 = s.inc_colpos[4 * k + 2] | (static_cast<uint32_t>(s.inc_colpos[4 * k + 3]) << 16);
-            inc[k].pad = 0;
-        }
-        upload(h->d_inc, inc.data(), inc.size() * sizeof(IncWide));
-    }
+    upload(h->d_inc, words.data(), words.size() * sizeof(uint32_t));
     h->d_scalars.ensure(64);
     if (h->seg_vbeg.empty())
         h->seg_vbeg = {0, s.num_vertices};
@@ -226,10 +229,10 @@ void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int w
         const size_t T = static_cast<size_t>(h->setup.num_tets);
         h->d_proj_slot.ensure(std::max<size_t>(1, T) * sizeof(int32_t));
         h->d_proj_list.ensure(std::max<size_t>(1, T) * sizeof(int32_t));
-        h->d_proj_M.ensure(std::max<size_t>(1, T) * 45 * sizeof(double));
+        h->d_proj_H.ensure(std::max<size_t>(1, T) * kProjPerTet * sizeof(double));
         a.proj_slot = h->d_proj_slot.as<int32_t>();
         a.proj_list = h->d_proj_list.as<int32_t>();
-        a.proj_M = h->d_proj_M.as<double>();
+        a.proj_H = h->d_proj_H.as<double>();
     }
     if (h->timing)
         hip_check(hipEventRecord(h->ev[0], stream), "hipEventRecord");

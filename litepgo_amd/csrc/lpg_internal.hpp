@@ -35,16 +35,18 @@ enum StatusBits : unsigned {
 //   conn[T]  : the 4 vertex ids of a tet as int4, 16 bytes per tet
 //   theta[T] : (lambda, mu) as double2, 16 bytes per tet
 
-// Incidence record of (vertex v, tet e): the tet id and, for each local slot b of e, the position of vertex
-// e[b] in v's sorted neighbour list (the column block of H in v's rows).  Narrow form packs 4 x uint8.
+// Incidence record of (vertex v, tet e), for v at slot a of e.  The kernels process the tet in the vertex order
+// (a, a^1, a^2, a^3) -- an even permutation of (0, 1, 2, 3), so det Dm and w keep their sign -- which puts v at
+// local slot 0.  The record holds the tet id, a, and for j = 1..3 the position of vertex e[a^j] in v's sorted
+// neighbour list (its column block in v's rows).  Narrow form: 3 x 8-bit positions and a in bits 24..25.
 struct IncNarrow {
     int32_t tet;
-    uint32_t colpos; // 4 x 8 bits
+    uint32_t packed; // cp1 | cp2 << 8 | cp3 << 16 | a << 24
 };
 struct IncWide {
     int32_t tet;
-    uint32_t colpos01; // 2 x 16 bits
-    uint32_t colpos23;
+    uint32_t cp12; // cp1 | cp2 << 16
+    uint32_t cp3a; // cp3 | a << 16
     uint32_t pad;
 };
 
@@ -63,7 +65,8 @@ struct MeshSetup {
     std::vector<int32_t> nbr;        // sorted vertex neighbours incl. self
     std::vector<int32_t> inc_ptr;    // num_vertices + 1
     std::vector<int32_t> inc_tet;    // 4 T, element order per vertex
-    std::vector<uint16_t> inc_colpos; // 4 T x 4
+    std::vector<uint16_t> inc_colpos; // 4 T x 4, position of e[b] in nbr(v) for slot b (original order)
+    std::vector<int32_t> self_pos;   // num_vertices, position of v in nbr(v)
     std::vector<int32_t> conn;       // T x 4 vertex ids
     std::vector<double> Xr;          // V x 4 rest positions (x, y, z, 0)
     std::vector<double> theta;       // T x 2 (lambda, mu) snapshot
@@ -101,6 +104,7 @@ struct DeviceMesh {
     const void* inc = nullptr; // IncNarrow* or IncWide*
     const int32_t* nbr_ptr = nullptr;
     const int32_t* nbr = nullptr;
+    const int32_t* self_pos = nullptr;
     const int32_t* row_ptr = nullptr;
     int32_t num_vertices = 0;
     int32_t num_tets = 0;
@@ -121,8 +125,13 @@ struct EvalArgs {
     unsigned* counter = nullptr; // indefinite count (device)
     int32_t* proj_slot = nullptr; // T
     int32_t* proj_list = nullptr; // T
-    double* proj_M = nullptr;    // T x 45
+    double* proj_H = nullptr;    // T x kProjPerTet (slot-indexed, first *counter used)
 };
+
+// Projected element blocks stored per indefinite tet: for each slot a, 6 distinct entries of the symmetric block
+// (a, a) and the three blocks (a, a^j), j = 1..3.
+constexpr int kProjPerSlot = 6 + 27;
+constexpr int kProjPerTet = 4 * kProjPerSlot;
 
 struct LaunchConfig {
     int lanes_per_vertex = 4; // lanes sharing one vertex's incidences

@@ -41,15 +41,9 @@ __device__ __forceinline__ double dot3(double a0, double a1, double a2, double b
 
 constexpr int kWave = 64;
 
-// Helmert basis of 1-perp in R^4: P(a, m).  kHelmertC is the same table as a compile-time constant (its zeros fold
-// away); kHelmert lives in constant memory for the kernels that index it at run time.
-constexpr double kHelmertC[4][3] = {
-    {0.70710678118654752440, 0.40824829046386301637, 0.28867513459481288225},
-    {-0.70710678118654752440, 0.40824829046386301637, 0.28867513459481288225},
-    {0.0, -0.81649658092772603273, 0.28867513459481288225},
-    {0.0, 0.0, -0.86602540378443864676},
-};
-__device__ __constant__ double kHelmert[4][3] = {
+// Helmert basis of 1-perp in R^4: P(a, m) (orthonormal columns orthogonal to (1, 1, 1, 1)).  Indexed with
+// compile-time indices only, so its zeros fold away.
+constexpr double kP[4][3] = {
     {0.70710678118654752440, 0.40824829046386301637, 0.28867513459481288225},
     {-0.70710678118654752440, 0.40824829046386301637, 0.28867513459481288225},
     {0.0, -0.81649658092772603273, 0.28867513459481288225},
@@ -199,20 +193,37 @@ template <bool GLOBAL_ACC> __device__ __forceinline__ void accum(double* p, doub
         __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Accessors of the incidence records: cp(r, j) is the column-block position of the vertex at rotated slot j = 1..3,
+// slot(r) the vertex's slot a in the tet's own order.
 template <bool WIDE> struct IncT;
 template <> struct IncT<false> {
     using type = IncNarrow;
-    __device__ static int colpos(const IncNarrow& r, int b) {
-        return (r.colpos >> (8 * b)) & 0xff;
+    __device__ static int cp(const IncNarrow& r, int j) {
+        return (r.packed >> (8 * (j - 1))) & 0xff;
+    }
+    __device__ static int slot(const IncNarrow& r) {
+        return (r.packed >> 24) & 3;
     }
 };
 template <> struct IncT<true> {
     using type = IncWide;
-    __device__ static int colpos(const IncWide& r, int b) {
-        const uint32_t w = b < 2 ? r.colpos01 : r.colpos23;
-        return (w >> (16 * (b & 1))) & 0xffff;
+    __device__ static int cp(const IncWide& r, int j) {
+        return j == 1 ? (r.cp12 & 0xffff) : j == 2 ? (r.cp12 >> 16) : (r.cp3a & 0xffff);
+    }
+    __device__ static int slot(const IncWide& r) {
+        return (r.cp3a >> 16) & 3;
     }
 };
+
+// 1 / d from the hardware reciprocal estimate refined by two Newton steps (within an ulp of the quotient for the
+// normal range used here; 0 gives NaN and the caller's finiteness checks fire).
+__device__ __forceinline__ double recip(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
 
 // Row `a` of a 4x3 register array as a masked sum.  A select chain here is rewritten by the compiler into a
 // dynamically indexed private array (scratch memory); multiplying by 0.0 / 1.0 masks keeps it in registers.
@@ -225,12 +236,14 @@ __device__ __forceinline__ void select_row(const double G[4][3], int a, double o
 
 // ---------------------------------------------------------------------------------------------------------------
 // Assembly kernel: one workgroup per vertex tile.  The LPV lanes of a vertex split its incidence list (lane s takes
-// incidences s, s + LPV, s + 2 LPV, ...); each lane computes the three rows of its tet's element Hessian that belong
-// to the vertex and adds them to the vertex's H segment in LDS with ds_add_f64.  All lanes add at once: the order
-// in which the LPV contributions of one round land on a shared value is fixed by the hardware's handling of one
-// LDS atomic instruction, so results are reproducible run to run for a given LPV (and independent of the number of
-// waves, tiles and grid), and agree across LPV to rounding.  Gradient and energy contributions are summed per lane
-// in registers and combined in lane order at the end.  KIND: -1 none, 0 Exact, 1 Projected.
+// incidences s, s + LPV, s + 2 LPV, ...).  A lane evaluates its tet in the vertex order (a, a^1, a^2, a^3), which
+// puts the vertex at local slot 0, so the three rows of the element Hessian that belong to the vertex are the
+// blocks H_0j.  The diagonal block H_00 and the gradient and energy contributions are summed per lane in registers
+// and combined in lane order at the end of the tile; the off-diagonal blocks H_0j (j = 1..3) are added to the
+// vertex's H segment in LDS with ds_add_f64 by all lanes at once.  Lanes of one vertex collide on an address only
+// when their tets share the neighbour at the same rotated slot; the order of such colliding adds within one LDS
+// atomic instruction is fixed by the hardware, so results are reproducible run to run for a given LPV and agree
+// across launch shapes to rounding.  KIND: -1 none, 0 Exact, 1 Projected.
 // ---------------------------------------------------------------------------------------------------------------
 struct AssembleParams {
     const double* Xr;
@@ -239,7 +252,7 @@ struct AssembleParams {
     const void* inc;
     const int32_t* nbr_ptr;
     const int32_t* nbr;
-    const int32_t* row_ptr;
+    const int32_t* self_pos;
     const int32_t* tile_vbeg;
     const int32_t* tile_ids; // tiles to process (index into tile_vbeg), or null for 0..num_tiles-1
     const double* x;
@@ -249,15 +262,17 @@ struct AssembleParams {
     double* vertex_E; // per-vertex sum of the energies of the tets whose slot 0 it is
     unsigned* status;
     const int32_t* proj_slot;
-    const double* proj_M;
+    const double* proj_H;
 };
 
-// The 4 x 9 block-row contributions of one incidence (vertex = slot a of tet e), its gradient block and, for
-// slot 0, the tet's energy.  With Ds = [x1-x0 | x2-x0 | x3-x0] (deformed edges) and F = Ds Dm^-1, the vectors
-// q_b = F^-T g_b are the rows of Ds^-1 (b = 1..3) and q_0 = -(q_1 + q_2 + q_3); J = det Ds / det Dm.
+// The blocks H_0j (j = 1..3) and the 6 distinct entries of the symmetric H_00 of one incidence, its gradient block
+// and, when the vertex is the tet's slot 0 (a == 0), the tet's energy; t holds the tet in rotated order.  With
+// Ds = [x1-x0 | x2-x0 | x3-x0] (deformed edges) and F = Ds Dm^-1, the vectors q_j = F^-T g_j are the rows of Ds^-1
+// (j = 1..3) and q_0 = -(q_1 + q_2 + q_3); J = det Ds / det Dm.
 template <int KIND>
 __device__ __forceinline__ void incidence_values(const AssembleParams& p, const TetIn& t, int a, int e,
-                                                 double vals[4][9], double gc[3], double& ec, unsigned& flags) {
+                                                 double off[3][9], double diag[6], double gc[3], double& ec,
+                                                 unsigned& flags) {
     // Rest geometry.
     double Dm[9], Cm[9];
 #pragma unroll
@@ -273,7 +288,7 @@ __device__ __forceinline__ void incidence_values(const AssembleParams& p, const 
             Cm[3 * i + j] = fma(Dm[3 * i1 + j1], Dm[3 * i2 + j2], -(Dm[3 * i1 + j2] * Dm[3 * i2 + j1]));
         }
     const double det_m = dot3(Dm[0], Dm[1], Dm[2], Cm[0], Cm[1], Cm[2]);
-    const double inv_m = 1.0 / det_m;
+    const double inv_m = recip(det_m);
     const double w = det_m * (1.0 / 6.0);
     double G[4][3]; // rows of dN/dxi Dm^-1
 #pragma unroll
@@ -301,10 +316,10 @@ __device__ __forceinline__ void incidence_values(const AssembleParams& p, const 
             Cs[3 * i + j] = fma(Ds[3 * i1 + j1], Ds[3 * i2 + j2], -(Ds[3 * i1 + j2] * Ds[3 * i2 + j1]));
         }
     const double det_s = dot3(Ds[0], Ds[1], Ds[2], Cs[0], Cs[1], Cs[2]);
-    const double inv_s = 1.0 / det_s;
+    const double inv_s = recip(det_s);
     const double J = det_s * inv_m;
     const double L = log(J);
-    if (!(J > 0.0 && isfinite(J) && isfinite(L)))
+    if (!(det_m > 0.0 && J > 0.0 && isfinite(J) && isfinite(L)))
         flags |= kStatusDetF;
     if (!theta_ok(t.lam, t.mu))
         flags |= kStatusBadTheta;
@@ -323,15 +338,13 @@ __device__ __forceinline__ void incidence_values(const AssembleParams& p, const 
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            F[3 * i + j] = fma(Du[3 * i + 2], G[3][j], fma(Du[3 * i + 1], G[2][j], fma(Du[3 * i], G[1][j], i == j ? 1.0 : 0.0)));
-    double ga[3], qa[3];
-    select_row(G, a, ga);
-    select_row(q, a, qa);
+            F[3 * i + j] =
+                fma(Du[3 * i + 2], G[3][j], fma(Du[3 * i + 1], G[2][j], fma(Du[3 * i], G[1][j], i == j ? 1.0 : 0.0)));
     const double p_ = fma(t.lam, L, -t.mu);
-    // Gradient block w (mu F g_a + p q_a).
+    // Gradient block w (mu F g_0 + p q_0).
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-        gc[i] = w * fma(t.mu, dot3(F[3 * i], F[3 * i + 1], F[3 * i + 2], ga[0], ga[1], ga[2]), p_ * qa[i]);
+        gc[i] = w * fma(t.mu, dot3(F[3 * i], F[3 * i + 1], F[3 * i + 2], G[0][0], G[0][1], G[0][2]), p_ * q[0][i]);
     if (a == 0) {
         double sq = 0.0;
 #pragma unroll
@@ -343,69 +356,48 @@ __device__ __forceinline__ void incidence_values(const AssembleParams& p, const 
         return;
     const int slot = KIND == 1 ? p.proj_slot[e] : -1;
     if (slot < 0) {
-        // H_ab = w [mu (g_a . g_b) I + lambda q_a q_b^T - p q_b q_a^T]
+        // H_0j = w [mu (g_0 . g_j) I + lambda q_0 q_j^T - p q_j q_0^T];  H_00 = w [mu |g_0|^2 I + (lambda - p) q_0 q_0^T]
         const double sw = w * t.mu;
-        double xa[3], ya[3];
+        double xa[3], ya[3], ca[3];
+        const double c2 = w * (t.lam - p_);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            xa[i] = (w * t.lam) * qa[i];
-            ya[i] = (w * p_) * qa[i];
+            xa[i] = (w * t.lam) * q[0][i];
+            ya[i] = (w * p_) * q[0][i];
+            ca[i] = c2 * q[0][i];
         }
+        const double c1 = sw * dot3(G[0][0], G[0][1], G[0][2], G[0][0], G[0][1], G[0][2]);
+        diag[0] = fma(ca[0], q[0][0], c1);
+        diag[1] = ca[0] * q[0][1];
+        diag[2] = ca[0] * q[0][2];
+        diag[3] = fma(ca[1], q[0][1], c1);
+        diag[4] = ca[1] * q[0][2];
+        diag[5] = fma(ca[2], q[0][2], c1);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const double s1 = sw * dot3(ga[0], ga[1], ga[2], G[b][0], G[b][1], G[b][2]);
+        for (int j = 1; j < 4; ++j) {
+            const double s1 = sw * dot3(G[0][0], G[0][1], G[0][2], G[j][0], G[j][1], G[j][2]);
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
                 for (int kk = 0; kk < 3; ++kk) {
-                    double val = fma(xa[i], q[b][kk], -(q[b][i] * ya[kk]));
+                    double val = fma(xa[i], q[j][kk], -(q[j][i] * ya[kk]));
                     if (i == kk)
                         val += s1;
-                    vals[b][3 * i + kk] = val;
+                    off[j - 1][3 * i + kk] = val;
                 }
         }
     } else {
-        // proj(H)_ab = sum_mn P(a,m) P(b,n) Mp[(m,i),(n,k)], Mp packed upper-triangular row-major.
-        const double* Mp = p.proj_M + 45 * static_cast<size_t>(slot);
-        double Pa[3];
-        {
-            double Ph[4][3];
+        // Indefinite element: the eigen-clamp kernel stored, for each slot a, the 6 distinct entries of proj(H)_aa
+        // and the blocks proj(H)_{a, a^j}, j = 1..3.
+        const double* src = p.proj_H + static_cast<size_t>(slot) * kProjPerTet + kProjPerSlot * a;
 #pragma unroll
-            for (int b = 0; b < 4; ++b)
+        for (int c = 0; c < 6; ++c)
+            diag[c] = src[c];
 #pragma unroll
-                for (int m = 0; m < 3; ++m)
-                    Ph[b][m] = kHelmert[b][m];
-            select_row(Ph, a, Pa);
-        }
-        // R_n[i][k] = sum_m Pa[m] Mp[(3m+i), (3n+k)], vals[b] = sum_n P(b,n) R_n, one n at a time.
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
+        for (int j = 0; j < 3; ++j)
 #pragma unroll
             for (int c = 0; c < 9; ++c)
-                vals[b][c] = 0.0;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) {
-            double Rn[9];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int kk = 0; kk < 3; ++kk) {
-                    double s = 0.0;
-#pragma unroll
-                    for (int m = 0; m < 3; ++m) {
-                        const int rr = 3 * m + i, cc = 3 * n + kk;
-                        const int lo = rr < cc ? rr : cc, hi = rr < cc ? cc : rr;
-                        s = fma(Pa[m], Mp[lo * 9 - lo * (lo - 1) / 2 + (hi - lo)], s);
-                    }
-                    Rn[3 * i + kk] = s;
-                }
-#pragma unroll
-            for (int b = 0; b < 4; ++b)
-                if (kHelmertC[b][n] != 0.0)
-#pragma unroll
-                    for (int c = 0; c < 9; ++c)
-                        vals[b][c] = fma(kHelmertC[b][n], Rn[c], vals[b][c]);
-        }
+                off[j][c] = src[6 + 9 * j + c];
     }
 }
 
@@ -421,8 +413,10 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
     constexpr int kVerts = WAVES * kPerWave; // lanes kPerWave * LPV .. 63 of each wave stay idle
     const Inc* __restrict__ inc = static_cast<const Inc*>(p.inc);
     const int vb = p.tile_vbeg[tile], ve = p.tile_vbeg[tile + 1];
-    const int seg0 = KIND >= 0 ? p.row_ptr[3 * vb] : 0;
-    const int seg1 = KIND >= 0 ? p.row_ptr[3 * ve] : 0;
+    const int nb = p.nbr_ptr[vb];
+    // Rows 3v..3v+2 hold 9 |nbr(v)| values, so the tile's segment starts at 9 nbr_ptr[vb].
+    const int seg0 = KIND >= 0 ? 9 * nb : 0;
+    const int seg1 = KIND >= 0 ? 9 * p.nbr_ptr[ve] : 0;
     double* gl = lds;
     double* el = gl + 3 * kVerts;
     double* acc = GLOBAL_ACC ? p.H + seg0 : el + kVerts;
@@ -437,63 +431,129 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
     __syncthreads();
     unsigned flags = 0;
     double gs0 = 0.0, gs1 = 0.0, gs2 = 0.0, es = 0.0;
+    double ds[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double* row = acc;
+    int rowlen = 0;
     if (owner) {
-        if (sub == 0) {
-            const double* xv = p.x + 3 * static_cast<size_t>(v);
-            if (!(isfinite(xv[0]) && isfinite(xv[1]) && isfinite(xv[2])))
-                flags |= kStatusNonFiniteX;
-        }
-        const int rbase = KIND >= 0 ? p.row_ptr[3 * v] - seg0 : 0;
-        const int rowlen = KIND >= 0 ? p.row_ptr[3 * v + 1] - p.row_ptr[3 * v] : 0;
-        double* row = acc + rbase;
-        const int32_t* my_nbr = p.nbr + p.nbr_ptr[v];
+        const int nv0 = p.nbr_ptr[v];
+        rowlen = 3 * (p.nbr_ptr[v + 1] - nv0);
+        row = acc + 9 * (nv0 - nb);
+        const int32_t* my_nbr = p.nbr + nv0;
+        // The vertex's own rest and current position: slot 0 of every tet in its list.
+        const double4 Xv = reinterpret_cast<const double4*>(p.Xr)[v];
+        const double* xv = p.x + 3 * static_cast<size_t>(v);
+        const double uv0 = xv[0], uv1 = xv[1], uv2 = xv[2];
+        if (sub == 0 && !(isfinite(uv0) && isfinite(uv1) && isfinite(uv2)))
+            flags |= kStatusNonFiniteX;
         const int kb = p.inc_ptr[v], ke = p.inc_ptr[v + 1];
+        // Software pipeline: the record of the round after next and the neighbour ids of the next round are
+        // loaded while the current round computes, so a round waits for one gather (vertex data) instead of three
+        // dependent loads (record -> neighbour ids -> vertex data).
+        Inc r_cur{}, r_next{};
+        int id_cur[3] = {0, 0, 0};
+        {
+            const int k = kb + sub;
+            if (k < ke)
+                r_cur = inc[k];
+            if (k + LPV < ke)
+                r_next = inc[k + LPV];
+            if (k < ke)
+#pragma unroll
+                for (int j = 1; j < 4; ++j)
+                    id_cur[j - 1] = my_nbr[IncT<WIDE>::cp(r_cur, j)];
+        }
         for (int k0 = kb; k0 < ke; k0 += LPV) {
             const int k = k0 + sub;
             const bool active = k < ke;
-            double vals[4][9];
-            int cps[4] = {0, 0, 0, 0};
-            if (active) {
-                const Inc r = inc[k];
-                int ids[4];
+            Inc r_after{};
+            int id_next[3] = {0, 0, 0};
+            if (k + 2 * LPV < ke)
+                r_after = inc[k + 2 * LPV];
+            if (k + LPV < ke)
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    cps[b] = IncT<WIDE>::colpos(r, b);
-                    ids[b] = my_nbr[cps[b]];
+                for (int j = 1; j < 4; ++j)
+                    id_next[j - 1] = my_nbr[IncT<WIDE>::cp(r_next, j)];
+            const Inc r = r_cur;
+            double off[3][9];
+            int cps[3] = {0, 0, 0};
+            if (active) {
+                int ids[4];
+                ids[0] = v;
+#pragma unroll
+                for (int j = 1; j < 4; ++j) {
+                    cps[j - 1] = IncT<WIDE>::cp(r, j);
+                    ids[j] = id_cur[j - 1];
                 }
-                const int a = (ids[0] == v) ? 0 : (ids[1] == v) ? 1 : (ids[2] == v) ? 2 : 3;
                 TetIn t;
-                load_vertices(p.Xr, p.x, ids, t);
+                t.X[0][0] = Xv.x;
+                t.X[0][1] = Xv.y;
+                t.X[0][2] = Xv.z;
+                t.u[0][0] = uv0;
+                t.u[0][1] = uv1;
+                t.u[0][2] = uv2;
+#pragma unroll
+                for (int j = 1; j < 4; ++j) {
+                    const double4 X = reinterpret_cast<const double4*>(p.Xr)[ids[j]];
+                    t.X[j][0] = X.x;
+                    t.X[j][1] = X.y;
+                    t.X[j][2] = X.z;
+                    const double* pu = p.x + 3 * static_cast<size_t>(ids[j]);
+                    t.u[j][0] = pu[0];
+                    t.u[j][1] = pu[1];
+                    t.u[j][2] = pu[2];
+                }
                 load_theta(p.theta_snap, p.theta, r.tet, t);
-                double gc[3], ec = 0.0;
-                incidence_values<KIND>(p, t, a, r.tet, vals, gc, ec, flags);
+                double gc[3], ec = 0.0, dg[6];
+                incidence_values<KIND>(p, t, IncT<WIDE>::slot(r), r.tet, off, dg, gc, ec, flags);
                 gs0 += gc[0];
                 gs1 += gc[1];
                 gs2 += gc[2];
                 es += ec;
+                if (KIND >= 0)
+#pragma unroll
+                    for (int c = 0; c < 6; ++c)
+                        ds[c] += dg[c];
             }
             if (KIND >= 0) {
 #pragma unroll
                 for (int s = 0; s < (GLOBAL_ACC ? LPV : 1); ++s) {
                     if (active && (!GLOBAL_ACC || sub == s)) {
 #pragma unroll
-                        for (int b = 0; b < 4; ++b) {
-                            double* blk = row + 3 * cps[b];
+                        for (int j = 0; j < 3; ++j) {
+                            double* blk = row + 3 * cps[j];
 #pragma unroll
                             for (int i = 0; i < 3; ++i)
 #pragma unroll
                                 for (int kk = 0; kk < 3; ++kk)
-                                    accum<GLOBAL_ACC>(blk + i * rowlen + kk, vals[b][3 * i + kk]);
+                                    accum<GLOBAL_ACC>(blk + i * rowlen + kk, off[j][3 * i + kk]);
                         }
                     }
                 }
             }
+            r_cur = r_next;
+            r_next = r_after;
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                id_cur[j] = id_next[j];
         }
     }
-    // Per-vertex gradient and energy: the lanes' partial sums in lane order.
+    // Per-vertex diagonal block, gradient and energy: the lanes' partial sums in lane order.
+    const int self = owner ? p.self_pos[v] : 0;
 #pragma unroll
     for (int s = 0; s < LPV; ++s) {
         if (owner && sub == s) {
+            if (KIND >= 0) {
+                double* blk = row + 3 * self;
+                blk[0] += ds[0];
+                blk[1] += ds[1];
+                blk[2] += ds[2];
+                blk[rowlen] += ds[1];
+                blk[rowlen + 1] += ds[3];
+                blk[rowlen + 2] += ds[4];
+                blk[2 * rowlen] += ds[2];
+                blk[2 * rowlen + 1] += ds[4];
+                blk[2 * rowlen + 2] += ds[5];
+            }
             if (s == 0) {
                 gl[3 * lv] = gs0;
                 gl[3 * lv + 1] = gs1;
@@ -631,7 +691,7 @@ struct ProjectParams {
     unsigned* counter;
     int32_t* proj_slot;
     int32_t* proj_list;
-    double* proj_M;
+    double* proj_H;
 };
 
 __device__ __forceinline__ int sym_index(int r, int c) {
@@ -726,8 +786,8 @@ __global__ __launch_bounds__(64) void k_project_jacobi(ProjectParams p) {
         for (int m = 0; m < 3; ++m)
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-                cm[m][j] = fma(kHelmert[3][m], G[3][j],
-                               fma(kHelmert[2][m], G[2][j], fma(kHelmert[1][m], G[1][j], kHelmert[0][m] * G[0][j])));
+                cm[m][j] = fma(kP[3][m], G[3][j],
+                               fma(kP[2][m], G[2][j], fma(kP[1][m], G[1][j], kP[0][m] * G[0][j])));
 #pragma unroll
         for (int m = 0; m < 3; ++m)
 #pragma unroll
@@ -810,7 +870,7 @@ __global__ __launch_bounds__(64) void k_project_jacobi(ProjectParams p) {
 #pragma unroll
         for (int r = 0; r < 9; ++r)
             lam[r] = fmax(M[sym_index(r, r)], 0.0);
-        double* out = p.proj_M + 45 * static_cast<size_t>(idx);
+        // proj(M) = V max(Lambda, 0) V^T, packed upper (in place of M).
         bool fin = true;
 #pragma unroll
         for (int r = 0; r < 9; ++r)
@@ -820,9 +880,57 @@ __global__ __launch_bounds__(64) void k_project_jacobi(ProjectParams p) {
 #pragma unroll
                 for (int q = 0; q < 9; ++q)
                     s = fma(V[r][q] * lam[q], V[c][q], s);
-                out[sym_index(r, c)] = s;
+                M[sym_index(r, c)] = s;
                 fin = fin && isfinite(s);
             }
+        // proj(H)_ab = sum_mn P(a,m) P(b,n) proj(M)[(m,i),(n,k)].  For each slot a, store the 6 distinct entries of
+        // the symmetric block (a, a) and the blocks (a, a^j), j = 1..3 -- the values the incidence of the tet's
+        // vertex a adds to its rows.
+        double* out = p.proj_H + kProjPerTet * static_cast<size_t>(idx);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            double Ta[3][9]; // Ta[n][3i+k] = sum_m P(a,m) proj(M)[(3m+i), (3n+k)]
+#pragma unroll
+            for (int n = 0; n < 3; ++n)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int kk = 0; kk < 3; ++kk) {
+                        double s = 0.0;
+#pragma unroll
+                        for (int m = 0; m < 3; ++m)
+                            if (kP[a][m] != 0.0)
+                                s = fma(kP[a][m], M[sym_index(3 * m + i, 3 * n + kk)], s);
+                        Ta[n][3 * i + kk] = s;
+                    }
+            double* o = out + kProjPerSlot * a;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int b = a ^ j;
+                double blk[9];
+#pragma unroll
+                for (int c = 0; c < 9; ++c) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int n = 0; n < 3; ++n)
+                        if (kP[b][n] != 0.0)
+                            s = fma(kP[b][n], Ta[n][c], s);
+                    blk[c] = s;
+                }
+                if (j == 0) {
+                    o[0] = blk[0];
+                    o[1] = blk[1];
+                    o[2] = blk[2];
+                    o[3] = blk[4];
+                    o[4] = blk[5];
+                    o[5] = blk[8];
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 9; ++c)
+                        o[6 + 9 * (j - 1) + c] = blk[c];
+                }
+            }
+        }
         if (!converged || !fin)
             atomicOr(p.status, kStatusProjection);
     }
@@ -1118,7 +1226,7 @@ void launch_assemble(const DeviceMesh& m, const EvalArgs& a, const LaunchConfig&
     p.inc = m.inc;
     p.nbr_ptr = m.nbr_ptr;
     p.nbr = m.nbr;
-    p.row_ptr = m.row_ptr;
+    p.self_pos = m.self_pos;
     p.tile_vbeg = tile_vbeg;
     p.tile_ids = tile_ids;
     p.x = a.x;
@@ -1128,7 +1236,7 @@ void launch_assemble(const DeviceMesh& m, const EvalArgs& a, const LaunchConfig&
     p.vertex_E = a.vertex_E;
     p.status = a.status;
     p.proj_slot = a.proj_slot;
-    p.proj_M = a.proj_M;
+    p.proj_H = a.proj_H;
     const size_t bytes = assemble_lds_bytes(lc, caps, global_acc, a.kind);
     if (bytes > assemble_lds_limit(lc))
         fail(kLogic, "launch_assemble: staged tile data exceeds the LDS of a workgroup");
@@ -1163,7 +1271,7 @@ void launch_energy_reduce(const double* vertex_E, const int32_t* chunk_begin, in
 void launch_project_prepass(const DeviceMesh& m, const EvalArgs& a, hipStream_t stream) {
     if (m.num_tets == 0)
         return;
-    ProjectParams p{m.Xr, m.conn, m.theta, a.x, a.theta, m.num_tets, a.status, a.counter, a.proj_slot, a.proj_list, a.proj_M};
+    ProjectParams p{m.Xr, m.conn, m.theta, a.x, a.theta, m.num_tets, a.status, a.counter, a.proj_slot, a.proj_list, a.proj_H};
     hipLaunchKernelGGL(k_project_prepass, dim3((m.num_tets + 255) / 256), dim3(256), 0, stream, p);
     check_launch("k_project_prepass");
 }
@@ -1171,7 +1279,7 @@ void launch_project_prepass(const DeviceMesh& m, const EvalArgs& a, hipStream_t 
 void launch_project_jacobi(const DeviceMesh& m, const EvalArgs& a, hipStream_t stream) {
     if (m.num_tets == 0)
         return;
-    ProjectParams p{m.Xr, m.conn, m.theta, a.x, a.theta, m.num_tets, a.status, a.counter, a.proj_slot, a.proj_list, a.proj_M};
+    ProjectParams p{m.Xr, m.conn, m.theta, a.x, a.theta, m.num_tets, a.status, a.counter, a.proj_slot, a.proj_list, a.proj_H};
     const int blocks = std::min(4096, (m.num_tets + 63) / 64);
     hipLaunchKernelGGL(k_project_jacobi, dim3(blocks), dim3(64), 0, stream, p);
     check_launch("k_project_jacobi");

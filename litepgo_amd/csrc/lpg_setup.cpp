@@ -214,11 +214,13 @@ void build_mesh_setup(MeshSetup& s, int32_t nv, int32_t nt, const int32_t* tets,
         }
     });
 
-    // Column-block positions per incidence.
+    // Column-block positions per incidence, and each vertex's own position in its neighbour list.
     s.inc_colpos.resize(16 * static_cast<size_t>(nt));
+    s.self_pos.assign(static_cast<size_t>(nv), 0);
     parallel_for(nv, nthreads, [&](int64_t v) {
         const int32_t* nb_begin = s.nbr.data() + s.nbr_ptr[v];
         const int32_t* nb_end = s.nbr.data() + s.nbr_ptr[v + 1];
+        s.self_pos[v] = static_cast<int32_t>(std::lower_bound(nb_begin, nb_end, static_cast<int32_t>(v)) - nb_begin);
         for (int32_t k = s.inc_ptr[v]; k < s.inc_ptr[v + 1]; ++k) {
             const int64_t e = s.inc_tet[k];
             for (int b = 0; b < 4; ++b)
