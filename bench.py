@@ -1,14 +1,14 @@
 #!/usr/bin/env python3
 """Benchmark of the neo-Hookean Tet P1 assembly on one MI355X.
 
-One step = one Energy::eval(x, gradient=True, hessian=Exact) of the 64^3 Kuhn-grid workload (1,572,864 tets,
+One step = one Energy::eval(x, gradient=True, hessian=Projected) of the 64^3 Kuhn-grid workload (1,572,864 tets,
 274,625 vertices, 36,168,777 Hessian values; E = 1e5, nu = 0.45, 20% stretch in x plus U(+-0.02 h) noise): energy,
-gradient and the assembled CSR Hessian values, with inputs and outputs resident in HBM.  `value` is tets/s of the
-whole step (kernel launches, the status/energy read-back and the host synchronisation included).
+gradient and the PSD-projected Hessian assembled into the CSR value array, with inputs and outputs resident in HBM.
+`value` is tets/s of the whole step (every kernel, the status/energy read-back and the host synchronisation).
 
-Also reported: the assembly kernel against the HBM roofline (HIP events around the kernel on its stream), the
-Projected-Hessian step, and the oracle (C++ CPU restatement of the reference algorithm) timed on this host's
-cores on a bounded sample.
+Also reported: the assembly kernel against the HBM roofline (HIP events around the kernel on its stream) and its
+FP64 rate against the FP64 vector peak, the per-phase times, the same step with the Exact Hessian, and the oracle
+(C++ CPU restatement of the reference algorithm) timed on this host's cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -32,37 +32,52 @@ FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector, vendor figure
 # Algorithmic (compulsory) bytes per assembly: 176 B per tet (connectivity 16, Dm^-1 72, w 8, (lambda, mu) 16,
 # 16 int32 block slots 64), 48 B per vertex (x read, g write), 8 B per Hessian value written.
 BYTES_PER_TET, BYTES_PER_VERTEX, BYTES_PER_NNZ = 176, 48, 8
+PMC_SUMMARY = ROOT / "profiles" / "assemble_projected_64cube_pmc.txt"
 
 
 def algorithmic_bytes(T, V, nnz):
     return BYTES_PER_TET * T + BYTES_PER_VERTEX * V + BYTES_PER_NNZ * nnz
 
 
-def read_traffic(path):
-    """HBM bytes per assembly launch from a committed rocprofv3 --pmc summary, or None."""
+def read_summary_value(path, key):
+    """A `key: value` figure from a committed rocprofv3 summary, or None."""
     if not path.exists():
         return None
-    text = path.read_text()
-    m = re.search(r"traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", text)
+    m = re.search(rf"^{key}\s*[:=]\s*([0-9.eE+]+)", path.read_text(), re.M)
     return float(m.group(1)) if m else None
 
 
 def cpu_baseline(w, nthreads, repeats):
-    """The oracle (oracle/liblpg_oracle.so) on the same 64^3 workload, E+g+H(Exact), median of `repeats`."""
+    """The oracle (oracle/liblpg_oracle.so) on the full 64^3 workload, E+g+H(Projected), median of `repeats`."""
     from oracle import oracle
 
     oracle.build()
     ref = oracle.OracleEnergy(w.num_vertices, w.tets, w.rest, w.theta, nthreads=nthreads)
-    ref.eval(w.u, True, "exact", nthreads=nthreads)  # warm-up
     times = []
     for _ in range(repeats):
         t0 = time.perf_counter()
-        ref.eval(w.u, True, "exact", nthreads=nthreads)
+        ref.eval(w.u, True, "projected", nthreads=nthreads)
         times.append(time.perf_counter() - t0)
     med = statistics.median(times)
     return {"value": w.tets.shape[0] / med, "unit": "tets/s", "cores": nthreads, "kind": "port",
-            "sample": f"oracle E+g+H(Exact)+CSR on the full 64^3 workload, median of {repeats} evals after 1 "
-                      f"warm-up, {nthreads} threads ({med * 1e3:.1f} ms per eval)"}
+            "sample": f"oracle E+g+H(Projected)+CSR on the full 64^3 workload (1,572,864 tets), median of {repeats} "
+                      f"evals, {nthreads} threads ({med:.2f} s per eval)"}
+
+
+def time_steps(dev, step, kind, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        step(kind)
+    dev.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(kind)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms, evals = dev.kernel_times()
+    return elapsed / steps * 1e3, [m / evals for m in ms]
 
 
 def main():
@@ -71,7 +86,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-repeats", type=int, default=5)
+    ap.add_argument("--cpu-repeats", type=int, default=3)
+    ap.add_argument("--exact-only", action="store_true", help="time only the Exact-Hessian step (profiling aid)")
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a known-size kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -81,7 +97,6 @@ def main():
               "in DESIGN.md.", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     from litepgo_amd import VolumetricFemEnergy, workloads
@@ -105,59 +120,47 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    for _ in range(args.warmup):
-        step("exact")
-    dev.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step("exact")
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_kernels, evals = dev.kernel_times()
-    ms_per_step = elapsed / args.steps * 1e3
-    assemble_ms = ms_kernels[1] / evals
     B = algorithmic_bytes(T, V, nnz)
-    achieved = B / (assemble_ms * 1e-3) / 1e9
-
-    # Projected Hessian, same workload.
-    for _ in range(2):
-        step("projected")
-    dev.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    psteps = max(5, args.steps // 2)
-    for _ in range(psteps):
-        step("projected")
-    torch.cuda.synchronize()
-    p_elapsed = time.perf_counter() - t0
-    p_ms, p_evals = dev.kernel_times()
+    e_ms, e_phase = time_steps(dev, step, "exact", args.steps, args.warmup)
+    exact = {"tets_per_s": T / (e_ms * 1e-3), "ms_per_step": e_ms, "assemble_ms": e_phase[1],
+             "assemble_hbm_frac": B / (e_phase[1] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if args.exact_only:
+        print(json.dumps({"exact": exact}), flush=True)
+        return
+    p_ms, p_phase = time_steps(dev, step, "projected", args.steps, args.warmup)
     indefinite = dev.last_indefinite_count
+    assemble_ms = p_phase[1]
+    achieved = B / (assemble_ms * 1e-3) / 1e9
+    flops = read_summary_value(PMC_SUMMARY, "fp64_flops_per_launch")
 
     result = {
         "metric": METRIC,
-        "value": T / (ms_per_step * 1e-3),
+        "value": T / (p_ms * 1e-3),
         "unit": "tets/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": p_ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "plain 64^3 Kuhn grid, E+g+H(Exact)+CSR assembly, E=1e5 nu=0.45, 20% x-stretch "
-                               "+ U(+-0.02h)", "tets": T, "vertices": V, "hessian_nnz": nnz,
+        "config": {"workload": "plain 64^3 Kuhn grid, E+g+H(PSD-projected)+CSR assembly, E=1e5 nu=0.45, "
+                               "20% x-stretch + U(+-0.02h)", "tets": T, "vertices": V, "hessian_nnz": nnz,
                    "launch": list(dev.get_launch())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": read_traffic(ROOT / "profiles" / "assemble_exact_64cube_pmc.txt"),
-                     "kernel": "k_assemble (Exact, gradient)", "kernel_ms": assemble_ms,
+                     "traffic": read_summary_value(PMC_SUMMARY, "traffic_bytes_per_launch"),
+                     "kernel": "k_assemble (Projected, gradient)", "kernel_ms": assemble_ms,
                      "algorithmic_bytes": B},
-        "projected": {"tets_per_s": T / (p_elapsed / psteps), "ms_per_step": p_elapsed / psteps * 1e3,
-                      "assemble_ms": p_ms[1] / p_evals, "projection_ms": p_ms[0] / p_evals,
-                      "indefinite_tets": indefinite, "indefinite_fraction": indefinite / T},
+        "fp64": None if flops is None else {
+            "achieved": flops / (assemble_ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": flops / (assemble_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, "flops_per_launch": flops},
+        "phases_ms": {"psd_test_and_eigen_clamp": p_phase[0], "assembly": p_phase[1],
+                      "energy_reduction": p_phase[2], "all_kernels": p_phase[3]},
+        "indefinite_tets": indefinite, "indefinite_fraction": indefinite / T,
+        "exact": exact,
     }
     if not args.no_cpu:
         nthreads = min(16, len(os.sched_getaffinity(0)))

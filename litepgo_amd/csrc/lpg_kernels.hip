@@ -24,6 +24,7 @@
 // exactly when A is.  A prepass tests A with an LDL^T factorization; only elements with an indefinite A get a
 // 9x9 Jacobi eigen-clamp of M, and the assembly reconstructs their blocks from proj(M).
 #include "lpg_internal.hpp"
+#include "lpg_eig9.hpp"
 
 #include <cmath>
 
@@ -215,6 +216,20 @@ template <> struct IncT<true> {
     }
 };
 
+// 1 / sqrt(x) and sqrt(x) for finite x > 0 from the hardware estimate refined by Newton steps (within an ulp or
+// so; only the eigen-clamp's rotation angles use them, where c^2 + s^2 = 1 to rounding is what matters).
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    y = y * fma(-hx * y, y, 1.5);
+    return y * fma(-hx * y, y, 1.5);
+}
+__device__ __forceinline__ double sqrt_nr(double x) {
+    const double y = rsqrt_nr(x);
+    const double r = x * y;
+    return fma(0.5 * y, fma(-r, r, x), r);
+}
+
 // 1 / d from the hardware reciprocal estimate refined by two Newton steps (within an ulp of the quotient for the
 // normal range used here; 0 gives NaN and the caller's finiteness checks fire).
 __device__ __forceinline__ double recip(double d) {
@@ -265,14 +280,14 @@ struct AssembleParams {
     const double* proj_H;
 };
 
-// The blocks H_0j (j = 1..3) and the 6 distinct entries of the symmetric H_00 of one incidence, its gradient block
-// and, when the vertex is the tet's slot 0 (a == 0), the tet's energy; t holds the tet in rotated order.  With
+// The exact blocks H_0j (j = 1..3) and the 6 distinct entries of the symmetric H_00 of one incidence (KIND >= 0),
+// its gradient block and, when the vertex is the tet's slot 0 (a == 0), the tet's energy; t holds the tet in
+// rotated order.  With
 // Ds = [x1-x0 | x2-x0 | x3-x0] (deformed edges) and F = Ds Dm^-1, the vectors q_j = F^-T g_j are the rows of Ds^-1
 // (j = 1..3) and q_0 = -(q_1 + q_2 + q_3); J = det Ds / det Dm.
 template <int KIND>
-__device__ __forceinline__ void incidence_values(const AssembleParams& p, const TetIn& t, int a, int e,
-                                                 double off[3][9], double diag[6], double gc[3], double& ec,
-                                                 unsigned& flags) {
+__device__ __forceinline__ void incidence_values(const TetIn& t, int a, double off[3][9], double diag[6],
+                                                 double gc[3], double& ec, unsigned& flags) {
     // Rest geometry.
     double Dm[9], Cm[9];
 #pragma unroll
@@ -354,8 +369,7 @@ __device__ __forceinline__ void incidence_values(const AssembleParams& p, const 
     }
     if (KIND < 0)
         return;
-    const int slot = KIND == 1 ? p.proj_slot[e] : -1;
-    if (slot < 0) {
+    {
         // H_0j = w [mu (g_0 . g_j) I + lambda q_0 q_j^T - p q_j q_0^T];  H_00 = w [mu |g_0|^2 I + (lambda - p) q_0 q_0^T]
         const double sw = w * t.mu;
         double xa[3], ya[3], ca[3];
@@ -386,18 +400,6 @@ __device__ __forceinline__ void incidence_values(const AssembleParams& p, const 
                     off[j - 1][3 * i + kk] = val;
                 }
         }
-    } else {
-        // Indefinite element: the eigen-clamp kernel stored, for each slot a, the 6 distinct entries of proj(H)_aa
-        // and the blocks proj(H)_{a, a^j}, j = 1..3.
-        const double* src = p.proj_H + static_cast<size_t>(slot) * kProjPerTet + kProjPerSlot * a;
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-            diag[c] = src[c];
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int c = 0; c < 9; ++c)
-                off[j][c] = src[6 + 9 * j + c];
     }
 }
 
@@ -451,31 +453,45 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
         // dependent loads (record -> neighbour ids -> vertex data).
         Inc r_cur{}, r_next{};
         int id_cur[3] = {0, 0, 0};
+        int slot_cur = -1; // projection slot of the current incidence's tet (Projected)
         {
             const int k = kb + sub;
             if (k < ke)
                 r_cur = inc[k];
             if (k + LPV < ke)
                 r_next = inc[k + LPV];
-            if (k < ke)
+            if (k < ke) {
 #pragma unroll
                 for (int j = 1; j < 4; ++j)
                     id_cur[j - 1] = my_nbr[IncT<WIDE>::cp(r_cur, j)];
+                if (KIND == 1)
+                    slot_cur = p.proj_slot[r_cur.tet];
+            }
         }
+        // Indefinite incidences met in this pass (Projected): the records and projection slots of up to two per
+        // lane are kept for the second pass; more set `pend_all`, which rescans the lane's incidences.
+        Inc pend_r0{}, pend_r1{};
+        int pend_s0 = -1, pend_s1 = -1;
+        bool pend_all = false;
         for (int k0 = kb; k0 < ke; k0 += LPV) {
             const int k = k0 + sub;
             const bool active = k < ke;
             Inc r_after{};
             int id_next[3] = {0, 0, 0};
+            int slot_next = -1;
             if (k + 2 * LPV < ke)
                 r_after = inc[k + 2 * LPV];
-            if (k + LPV < ke)
+            if (k + LPV < ke) {
 #pragma unroll
                 for (int j = 1; j < 4; ++j)
                     id_next[j - 1] = my_nbr[IncT<WIDE>::cp(r_next, j)];
+                if (KIND == 1)
+                    slot_next = p.proj_slot[r_next.tet];
+            }
             const Inc r = r_cur;
             double off[3][9];
             int cps[3] = {0, 0, 0};
+            bool add = false;
             if (active) {
                 int ids[4];
                 ids[0] = v;
@@ -504,12 +520,25 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
                 }
                 load_theta(p.theta_snap, p.theta, r.tet, t);
                 double gc[3], ec = 0.0, dg[6];
-                incidence_values<KIND>(p, t, IncT<WIDE>::slot(r), r.tet, off, dg, gc, ec, flags);
+                incidence_values<KIND>(t, IncT<WIDE>::slot(r), off, dg, gc, ec, flags);
                 gs0 += gc[0];
                 gs1 += gc[1];
                 gs2 += gc[2];
                 es += ec;
-                if (KIND >= 0)
+                // Indefinite tets (Projected) are added by the second pass below.
+                add = KIND == 0 || (KIND == 1 && slot_cur < 0);
+                if (KIND == 1 && slot_cur >= 0) {
+                    if (pend_s0 < 0) {
+                        pend_s0 = slot_cur;
+                        pend_r0 = r;
+                    } else if (pend_s1 < 0) {
+                        pend_s1 = slot_cur;
+                        pend_r1 = r;
+                    } else {
+                        pend_all = true;
+                    }
+                }
+                if (add)
 #pragma unroll
                     for (int c = 0; c < 6; ++c)
                         ds[c] += dg[c];
@@ -517,7 +546,7 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
             if (KIND >= 0) {
 #pragma unroll
                 for (int s = 0; s < (GLOBAL_ACC ? LPV : 1); ++s) {
-                    if (active && (!GLOBAL_ACC || sub == s)) {
+                    if (add && (!GLOBAL_ACC || sub == s)) {
 #pragma unroll
                         for (int j = 0; j < 3; ++j) {
                             double* blk = row + 3 * cps[j];
@@ -532,9 +561,62 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
             }
             r_cur = r_next;
             r_next = r_after;
+            slot_cur = slot_next;
 #pragma unroll
             for (int j = 0; j < 3; ++j)
                 id_cur[j] = id_next[j];
+        }
+        if (KIND == 1) {
+            // Second pass: indefinite tets add proj(H_e) as stored by the eigen-clamp kernel (for slot a: the 6
+            // distinct entries of block (a, a), then blocks (a, a^j), j = 1..3).  Keeping these loads out of the
+            // first pass keeps them from sharing registers with the exact values there.
+            const int steps = pend_all ? (ke - kb + LPV - 1) / LPV : 2;
+            for (int q = 0; q < steps; ++q) {
+                bool add = false;
+                int cps[3] = {0, 0, 0};
+                const double* src = nullptr;
+                Inc r = q == 0 ? pend_r0 : pend_r1;
+                int slot = q == 0 ? pend_s0 : pend_s1;
+                if (pend_all) {
+                    const int k = kb + sub + q * LPV;
+                    slot = -1;
+                    if (k < ke) {
+                        r = inc[k];
+                        slot = p.proj_slot[r.tet];
+                    }
+                }
+                double pv[kProjPerSlot];
+                if (slot >= 0) {
+                    add = true;
+#pragma unroll
+                    for (int j = 1; j < 4; ++j)
+                        cps[j - 1] = IncT<WIDE>::cp(r, j);
+                    src = p.proj_H + static_cast<size_t>(slot) * kProjPerTet + kProjPerSlot * IncT<WIDE>::slot(r);
+                    // All 33 loads are issued before any add (the empty asm orders them), so the entry costs one
+                    // memory latency rather than one per small group of loads.
+#pragma unroll
+                    for (int c = 0; c < kProjPerSlot; ++c)
+                        pv[c] = src[c];
+                    asm volatile("" ::: "memory");
+#pragma unroll
+                    for (int c = 0; c < 6; ++c)
+                        ds[c] += pv[c];
+                }
+#pragma unroll
+                for (int s = 0; s < (GLOBAL_ACC ? LPV : 1); ++s) {
+                    if (add && (!GLOBAL_ACC || sub == s)) {
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) {
+                            double* blk = row + 3 * cps[j];
+#pragma unroll
+                            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                                for (int kk = 0; kk < 3; ++kk)
+                                    accum<GLOBAL_ACC>(blk + i * rowlen + kk, pv[6 + 9 * j + 3 * i + kk]);
+                        }
+                    }
+                }
+            }
         }
     }
     // Per-vertex diagonal block, gradient and energy: the lanes' partial sums in lane order.
@@ -604,8 +686,14 @@ __device__ __forceinline__ int xcd_slot(int b, int G) {
     return x * q + (x < r ? x : r) + i;
 }
 
+// Waves per SIMD the register allocation must allow (168 VGPRs for 3): the common shapes fit without spilling;
+// one lane per vertex with a Projected Hessian and the HBM-accumulation tiles keep a 256-VGPR budget.
+constexpr int assemble_min_waves(int lpv, int kind, bool global_acc) {
+    return (global_acc || (lpv == 1 && kind == 1)) ? 2 : 3;
+}
+
 template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GLOBAL_ACC>
-__global__ __launch_bounds__(kWave* WAVES, KIND == 1 ? 2 : 3) void k_assemble(AssembleParams p) {
+__global__ __launch_bounds__(kWave* WAVES, assemble_min_waves(LPV, KIND, GLOBAL_ACC)) void k_assemble(AssembleParams p) {
     extern __shared__ double lds[];
     const int slot = xcd_slot(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
     const int tile = p.tile_ids ? p.tile_ids[slot] : slot;
@@ -614,7 +702,7 @@ __global__ __launch_bounds__(kWave* WAVES, KIND == 1 ? 2 : 3) void k_assemble(As
 
 // Persistent form: a fixed grid walks the tile list with a grid stride (the batched launch).
 template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GLOBAL_ACC>
-__global__ __launch_bounds__(kWave* WAVES, KIND == 1 ? 2 : 3) void k_assemble_persistent(AssembleParams p, int num_tiles) {
+__global__ __launch_bounds__(kWave* WAVES, 2) void k_assemble_persistent(AssembleParams p, int num_tiles) {
     extern __shared__ double lds[];
     for (int t = blockIdx.x; t < num_tiles; t += gridDim.x) {
         const int tile = p.tile_ids ? p.tile_ids[t] : t;
@@ -699,8 +787,12 @@ __device__ __forceinline__ int sym_index(int r, int c) {
     return lo * 9 - lo * (lo - 1) / 2 + (hi - lo);
 }
 
-__global__ __launch_bounds__(256) void k_project_prepass(ProjectParams p) {
-    const int e = blockIdx.x * 256 + threadIdx.x;
+constexpr int kPrepassThreads = 1024;
+
+__global__ __launch_bounds__(kPrepassThreads) void k_project_prepass(ProjectParams p) {
+    __shared__ unsigned wave_count[kPrepassThreads / kWave];
+    __shared__ unsigned block_base;
+    const int e = blockIdx.x * kPrepassThreads + threadIdx.x;
     bool indefinite = false;
     if (e < p.num_tets) {
         int ids[4];
@@ -734,7 +826,7 @@ __global__ __launch_bounds__(256) void k_project_prepass(ProjectParams p) {
             for (int j = 0; j < 9; ++j) {
                 const double d = A[sym_index(j, j)];
                 pd = pd && (d > 1e-13 * scale);
-                const double inv = 1.0 / d;
+                const double inv = recip(d);
 #pragma unroll
                 for (int i = j + 1; i < 9; ++i) {
                     const double lij = A[sym_index(j, i)] * inv;
@@ -746,20 +838,29 @@ __global__ __launch_bounds__(256) void k_project_prepass(ProjectParams p) {
             indefinite = !pd;
         }
     }
-    // One atomic per wave: the wave's indefinite lanes take consecutive slots in lane order.
+    // One global atomic per workgroup: the wave counts are combined in LDS, and the indefinite lanes take
+    // consecutive slots in (wave, lane) order after the workgroup's base.  A single counter hit once per wave
+    // serializes ~T/64 atomics at one L2 address; per workgroup it is 16x fewer.
     const unsigned long long mask = __ballot(indefinite);
+    const int lane = static_cast<int>(__lane_id());
+    const int wave = static_cast<int>(threadIdx.x) / kWave;
+    if (lane == 0)
+        wave_count[wave] = static_cast<unsigned>(__popcll(mask));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned total = 0;
+        for (int w = 0; w < kPrepassThreads / kWave; ++w)
+            total += wave_count[w];
+        block_base = total ? atomicAdd(p.counter, total) : 0u;
+    }
+    __syncthreads();
     int32_t slot = -1;
-    if (mask) {
-        const int lane = static_cast<int>(__lane_id());
-        const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-        unsigned base = 0;
-        if (lane == leader)
-            base = atomicAdd(p.counter, static_cast<unsigned>(__popcll(mask)));
-        base = __shfl(base, leader);
-        if (indefinite) {
-            slot = static_cast<int32_t>(base + __popcll(mask & ((1ull << lane) - 1ull)));
-            p.proj_list[slot] = e;
-        }
+    if (indefinite) {
+        unsigned base = block_base;
+        for (int w = 0; w < wave; ++w)
+            base += wave_count[w];
+        slot = static_cast<int32_t>(base + __popcll(mask & ((1ull << lane) - 1ull)));
+        p.proj_list[slot] = e;
     }
     if (e < p.num_tets)
         p.proj_slot[e] = slot;
@@ -805,71 +906,17 @@ __global__ __launch_bounds__(64) void k_project_jacobi(ProjectParams p) {
                     val = fma(t.mu, dot3(cm[m][0], cm[m][1], cm[m][2], cm[n][0], cm[n][1], cm[n][2]), val);
                 M[sym_index(r, c)] = geo.w * val;
             }
-        double V[9][9];
-#pragma unroll
-        for (int r = 0; r < 9; ++r)
-#pragma unroll
-            for (int c = 0; c < 9; ++c)
-                V[r][c] = (r == c) ? 1.0 : 0.0;
         double scale = 0.0;
 #pragma unroll
         for (int q = 0; q < 45; ++q)
             scale = fmax(scale, fabs(M[q]));
-        bool converged = false;
-        for (int sweep = 0; sweep < 20 && !converged; ++sweep) {
-            double off = 0.0;
-#pragma unroll
-            for (int r = 0; r < 9; ++r)
-#pragma unroll
-                for (int c = r + 1; c < 9; ++c)
-                    off = fmax(off, fabs(M[sym_index(r, c)]));
-            if (off <= 1e-15 * scale || off == 0.0) {
-                converged = true;
-                break;
-            }
-#pragma unroll
-            for (int pp = 0; pp < 9; ++pp)
-#pragma unroll
-                for (int qq = pp + 1; qq < 9; ++qq) {
-                    const double apq = M[sym_index(pp, qq)];
-                    if (fabs(apq) > 1e-18 * scale) {
-                        const double app = M[sym_index(pp, pp)], aqq = M[sym_index(qq, qq)];
-                        const double tau = (aqq - app) / (2.0 * apq);
-                        const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                        const double c = 1.0 / sqrt(1.0 + tt * tt), s = tt * c;
-#pragma unroll
-                        for (int r = 0; r < 9; ++r) {
-                            if (r == pp || r == qq)
-                                continue;
-                            const double mrp = M[sym_index(r, pp)], mrq = M[sym_index(r, qq)];
-                            M[sym_index(r, pp)] = fma(c, mrp, -(s * mrq));
-                            M[sym_index(r, qq)] = fma(s, mrp, c * mrq);
-                        }
-                        M[sym_index(pp, pp)] = fma(-tt, apq, app);
-                        M[sym_index(qq, qq)] = fma(tt, apq, aqq);
-                        M[sym_index(pp, qq)] = 0.0;
-#pragma unroll
-                        for (int r = 0; r < 9; ++r) {
-                            const double vrp = V[r][pp], vrq = V[r][qq];
-                            V[r][pp] = fma(c, vrp, -(s * vrq));
-                            V[r][qq] = fma(s, vrp, c * vrq);
-                        }
-                    }
-                }
-        }
-        if (!converged) {
-            double off = 0.0;
-#pragma unroll
-            for (int r = 0; r < 9; ++r)
-#pragma unroll
-                for (int c = r + 1; c < 9; ++c)
-                    off = fmax(off, fabs(M[sym_index(r, c)]));
-            converged = off <= 1e-13 * scale;
-        }
-        double lam[9];
+        // Eigen-decomposition by Householder tridiagonalization and implicit QL (in place: M is destroyed).
+        double lam[9], V[9][9];
+        const bool converged = eig9::sym_eig(
+            M, lam, V, scale, [](double x) { return rsqrt_nr(x); }, [](double x) { return recip(x); });
 #pragma unroll
         for (int r = 0; r < 9; ++r)
-            lam[r] = fmax(M[sym_index(r, r)], 0.0);
+            lam[r] = fmax(lam[r], 0.0);
         // proj(M) = V max(Lambda, 0) V^T, packed upper (in place of M).
         bool fin = true;
 #pragma unroll
@@ -1272,7 +1319,8 @@ void launch_project_prepass(const DeviceMesh& m, const EvalArgs& a, hipStream_t 
     if (m.num_tets == 0)
         return;
     ProjectParams p{m.Xr, m.conn, m.theta, a.x, a.theta, m.num_tets, a.status, a.counter, a.proj_slot, a.proj_list, a.proj_H};
-    hipLaunchKernelGGL(k_project_prepass, dim3((m.num_tets + 255) / 256), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(k_project_prepass, dim3((m.num_tets + kPrepassThreads - 1) / kPrepassThreads),
+                       dim3(kPrepassThreads), 0, stream, p);
     check_launch("k_project_prepass");
 }
 

@@ -1,0 +1,237 @@
+// CPU test of the 9x9 symmetric eigen-solver used by the PSD projection kernel (litepgo_amd/csrc/lpg_eig9.hpp):
+// orthonormal eigenvectors, reconstruction, and eigenvalues against a cyclic Jacobi reference, on random symmetric
+// matrices, matrices with repeated and zero eigenvalues, and reduced neo-Hookean element Hessians.
+#include "lpg_eig9.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <random>
+#include <vector>
+
+using lpg::eig9::N;
+using lpg::eig9::sym;
+
+namespace {
+
+double g_worst_orth = 0, g_worst_rec = 0, g_worst_eig = 0;
+
+void jacobi_eigenvalues(std::vector<double> A, std::vector<double>& ev) {
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0;
+        for (int p = 0; p < N; ++p)
+            for (int q = p + 1; q < N; ++q)
+                off += A[p * N + q] * A[p * N + q];
+        if (off < 1e-300)
+            break;
+        for (int p = 0; p < N; ++p)
+            for (int q = p + 1; q < N; ++q) {
+                const double apq = A[p * N + q];
+                if (apq == 0)
+                    continue;
+                const double tau = (A[q * N + q] - A[p * N + p]) / (2 * apq);
+                const double t = (tau >= 0 ? 1 : -1) / (std::fabs(tau) + std::sqrt(1 + tau * tau));
+                const double c = 1 / std::sqrt(1 + t * t), s = t * c;
+                for (int k = 0; k < N; ++k) {
+                    const double akp = A[k * N + p], akq = A[k * N + q];
+                    A[k * N + p] = c * akp - s * akq;
+                    A[k * N + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < N; ++k) {
+                    const double apk = A[p * N + k], aqk = A[q * N + k];
+                    A[p * N + k] = c * apk - s * aqk;
+                    A[q * N + k] = s * apk + c * aqk;
+                }
+            }
+    }
+    ev.resize(N);
+    for (int i = 0; i < N; ++i)
+        ev[i] = A[i * N + i];
+    std::sort(ev.begin(), ev.end());
+}
+
+bool check(const std::vector<double>& full, const char* what) {
+    double a[45], d[N], V[N][N];
+    double scale = 0;
+    for (int r = 0; r < N; ++r)
+        for (int c = r; c < N; ++c) {
+            a[sym(r, c)] = full[r * N + c];
+            scale = std::max(scale, std::fabs(full[r * N + c]));
+        }
+    const bool ok = lpg::eig9::sym_eig(
+        a, d, V, scale, [](double x) { return 1.0 / std::sqrt(x); }, [](double x) { return 1.0 / x; });
+    if (!ok) {
+        std::printf("FAIL %s: QL did not converge\n", what);
+        return false;
+    }
+    double orth = 0, rec = 0;
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+            double s = 0, m = 0;
+            for (int k = 0; k < N; ++k) {
+                s += V[k][i] * V[k][j];
+                m += V[i][k] * d[k] * V[j][k];
+            }
+            orth = std::max(orth, std::fabs(s - (i == j ? 1.0 : 0.0)));
+            rec = std::max(rec, std::fabs(m - full[i * N + j]));
+        }
+    std::vector<double> ref, got(d, d + N);
+    jacobi_eigenvalues(full, ref);
+    std::sort(got.begin(), got.end());
+    double eig = 0;
+    for (int i = 0; i < N; ++i)
+        eig = std::max(eig, std::fabs(got[i] - ref[i]));
+    rec /= scale;
+    eig /= scale;
+    g_worst_orth = std::max(g_worst_orth, orth);
+    g_worst_rec = std::max(g_worst_rec, rec);
+    g_worst_eig = std::max(g_worst_eig, eig);
+    if (orth > 1e-13 || rec > 1e-13 || eig > 1e-13 || !std::isfinite(orth + rec + eig)) {
+        std::printf("FAIL %s: orth %.3g rec %.3g eig %.3g\n", what, orth, rec, eig);
+        return false;
+    }
+    return true;
+}
+
+// Reduced element Hessian M = w (C (x) I3) A (C (x) I3)^T of neo-Hookean Tet P1 at deformation F (rest tet Dm).
+std::vector<double> neo_hookean_m(const double F[9], const double Dm[9], double lam, double mu) {
+    auto inv3 = [](const double* A, double* out) {
+        double C[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                C[3 * i + j] = A[3 * ((i + 1) % 3) + (j + 1) % 3] * A[3 * ((i + 2) % 3) + (j + 2) % 3] -
+                               A[3 * ((i + 1) % 3) + (j + 2) % 3] * A[3 * ((i + 2) % 3) + (j + 1) % 3];
+        const double det = A[0] * C[0] + A[1] * C[1] + A[2] * C[2];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                out[3 * i + j] = C[3 * j + i] / det;
+        return det;
+    };
+    double Fi[9], Di[9];
+    const double J = inv3(F, Fi);
+    const double detm = inv3(Dm, Di);
+    const double w = detm / 6, L = std::log(J), p = lam * L - mu;
+    double a[9]; // F^-T
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            a[3 * i + j] = Fi[3 * j + i];
+    double G[4][3];
+    for (int j = 0; j < 3; ++j) {
+        G[1][j] = Di[j];
+        G[2][j] = Di[3 + j];
+        G[3][j] = Di[6 + j];
+        G[0][j] = -(G[1][j] + G[2][j] + G[3][j]);
+    }
+    const double P[4][3] = {{0.70710678118654752440, 0.40824829046386301637, 0.28867513459481288225},
+                            {-0.70710678118654752440, 0.40824829046386301637, 0.28867513459481288225},
+                            {0.0, -0.81649658092772603273, 0.28867513459481288225},
+                            {0.0, 0.0, -0.86602540378443864676}};
+    double Cm[3][3]; // c_m = sum_a P(a,m) g_a
+    for (int m = 0; m < 3; ++m)
+        for (int j = 0; j < 3; ++j) {
+            Cm[m][j] = 0;
+            for (int b = 0; b < 4; ++b)
+                Cm[m][j] += P[b][m] * G[b][j];
+        }
+    std::vector<double> M(81);
+    for (int r = 0; r < 9; ++r)
+        for (int c = 0; c < 9; ++c) {
+            const int m = r / 3, i = r % 3, n = c / 3, k = c % 3;
+            double rm[3], rn[3];
+            for (int q = 0; q < 3; ++q) {
+                rm[q] = a[3 * q] * Cm[m][0] + a[3 * q + 1] * Cm[m][1] + a[3 * q + 2] * Cm[m][2];
+                rn[q] = a[3 * q] * Cm[n][0] + a[3 * q + 1] * Cm[n][1] + a[3 * q + 2] * Cm[n][2];
+            }
+            double v = lam * rm[i] * rn[k] - p * rn[i] * rm[k];
+            if (i == k)
+                v += mu * (Cm[m][0] * Cm[n][0] + Cm[m][1] * Cm[n][1] + Cm[m][2] * Cm[n][2]);
+            M[r * 9 + c] = w * v;
+        }
+    return M;
+}
+
+} // namespace
+
+int main() {
+    std::mt19937_64 rng(7);
+    std::uniform_real_distribution<double> U(-1, 1);
+    int fails = 0, cases = 0;
+    // Random symmetric matrices over several scales.
+    for (int t = 0; t < 2000; ++t) {
+        std::vector<double> A(81);
+        const double sc = std::pow(10.0, (t % 9) - 4);
+        for (int r = 0; r < N; ++r)
+            for (int c = r; c < N; ++c)
+                A[r * N + c] = A[c * N + r] = sc * U(rng);
+        fails += !check(A, "random");
+        ++cases;
+    }
+    // Repeated, zero and clustered eigenvalues: Q diag(lambda) Q^T with a random orthogonal Q.
+    for (int t = 0; t < 500; ++t) {
+        std::vector<double> Q(81), A(81, 0.0);
+        for (double& q : Q)
+            q = U(rng);
+        for (int j = 0; j < N; ++j) { // Gram-Schmidt on the columns
+            for (int i = 0; i < j; ++i) {
+                double s = 0;
+                for (int k = 0; k < N; ++k)
+                    s += Q[k * N + i] * Q[k * N + j];
+                for (int k = 0; k < N; ++k)
+                    Q[k * N + j] -= s * Q[k * N + i];
+            }
+            double n = 0;
+            for (int k = 0; k < N; ++k)
+                n += Q[k * N + j] * Q[k * N + j];
+            n = std::sqrt(n);
+            for (int k = 0; k < N; ++k)
+                Q[k * N + j] /= n;
+        }
+        double lam[N];
+        for (int i = 0; i < N; ++i)
+            lam[i] = (t % 3 == 0) ? (i < 4 ? 2.0 : -1.0) : (t % 3 == 1) ? (i < 3 ? 0.0 : U(rng)) : 1.0 + 1e-9 * U(rng);
+        for (int r = 0; r < N; ++r)
+            for (int c = 0; c < N; ++c)
+                for (int i = 0; i < N; ++i)
+                    A[r * N + c] += Q[r * N + i] * lam[i] * Q[c * N + i];
+        for (int r = 0; r < N; ++r)
+            for (int c = r + 1; c < N; ++c)
+                A[c * N + r] = A[r * N + c];
+        fails += !check(A, "spectrum");
+        ++cases;
+    }
+    // Already diagonal / tridiagonal inputs.
+    {
+        std::vector<double> A(81, 0.0);
+        for (int i = 0; i < N; ++i)
+            A[i * N + i] = i - 4.0;
+        fails += !check(A, "diagonal");
+        for (int i = 0; i + 1 < N; ++i)
+            A[i * N + i + 1] = A[(i + 1) * N + i] = 0.5;
+        fails += !check(A, "tridiagonal");
+        cases += 2;
+    }
+    // Reduced neo-Hookean element Hessians (E = 1e5, nu in [0.3, 0.49]) at random deformations, including strongly
+    // compressed and stretched states (indefinite tangents).
+    for (int t = 0; t < 3000; ++t) {
+        double Dm[9], F[9];
+        for (int q = 0; q < 9; ++q)
+            Dm[q] = (q % 4 == 0 ? 0.05 : 0.0) + 0.01 * U(rng);
+        const double amp = (t % 3 == 0) ? 0.05 : (t % 3 == 1) ? 0.3 : 0.6;
+        for (int q = 0; q < 9; ++q)
+            F[q] = (q % 4 == 0 ? 1.0 : 0.0) + amp * U(rng);
+        double det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                     F[2] * (F[3] * F[7] - F[4] * F[6]);
+        double detm = Dm[0] * (Dm[4] * Dm[8] - Dm[5] * Dm[7]) - Dm[1] * (Dm[3] * Dm[8] - Dm[5] * Dm[6]) +
+                      Dm[2] * (Dm[3] * Dm[7] - Dm[4] * Dm[6]);
+        if (det <= 0.05 || detm <= 0)
+            continue;
+        const double E = 1e5, nu = 0.3 + 0.19 * (0.5 + 0.5 * U(rng));
+        const double lam = E * nu / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
+        fails += !check(neo_hookean_m(F, Dm, lam, mu), "neo-Hookean");
+        ++cases;
+    }
+    std::printf("%d cases, %d failures; worst orthogonality %.3g, reconstruction %.3g, eigenvalues %.3g "
+                "(relative to max |M|)\n",
+                cases, fails, g_worst_orth, g_worst_rec, g_worst_eig);
+    return fails == 0 ? 0 : 1;
+}
