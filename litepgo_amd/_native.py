@@ -33,6 +33,12 @@ SIGNATURES = {
     "lpg_tet_energy_max_degree": (_I32, [_P]),
     "lpg_tet_energy_set_timing": (ctypes.c_int, [_P, _I32]),
     "lpg_tet_energy_kernel_times": (ctypes.c_int, [_P, _P, _P]),
+    "lpg_tet_batch_create": (ctypes.c_int, [_I32, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]),
+    "lpg_tet_batch_num_meshes": (_I32, [_P]),
+    "lpg_tet_batch_offsets": (ctypes.c_int, [_P, _P, _P, _P]),
+    "lpg_tet_batch_eval_device": (ctypes.c_int, [_P, _P, _P, _I32, _I32, _P, _P, _P, _P]),
+    "lpg_tet_energy_set_graph": (ctypes.c_int, [_P, _I32]),
+    "lpg_tet_energy_autotune": (ctypes.c_int, [_P, _P, _I32, _I32, _P, _P]),
 }
 
 LPG_OK = 0

@@ -10,7 +10,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "liblitepgo_fem.so"
-SOURCES = ["lpg_kernels.hip", "lpg_setup.cpp", "lpg_api.cpp"]
+SOURCES = ["lpg_kernels.hip", "lpg_setup.cpp", "lpg_api.cpp", "lpg_batch.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops stays off (hand-written FP64 code only; packed FP32 forms are avoided on these parts).

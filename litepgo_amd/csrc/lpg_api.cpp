@@ -55,6 +55,17 @@ BusyScope::~BusyScope() {
     flag.store(false);
 }
 
+void GraphCache::reset() {
+    if (exec)
+        (void)hipGraphExecDestroy(exec);
+    if (graph)
+        (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    key = GraphKey{};
+    pending = GraphKey{};
+}
+
 } // namespace lpg
 
 lpg::DeviceMesh lpg_tet_energy::mesh() const {
@@ -75,6 +86,7 @@ lpg::DeviceMesh lpg_tet_energy::mesh() const {
 }
 
 lpg_tet_energy::~lpg_tet_energy() {
+    graph.reset();
     for (lpg::DeviceBuffer* b : {&d_Xr, &d_conn, &d_theta, &d_inc_ptr, &d_inc, &d_nbr_ptr, &d_nbr, &d_self_pos, &d_row_ptr, &d_scalars, &d_vertex_E, &d_chunk_begin,
                                  &d_chunk_sum, &d_seg_chunk, &d_proj_slot, &d_proj_list, &d_proj_H, &d_x, &d_g, &d_H,
                                  &plan.d_tile_vbeg, &plan.d_normal_ids, &plan.d_oversized_ids})
@@ -90,9 +102,19 @@ lpg_tet_energy::~lpg_tet_energy() {
 
 namespace lpg {
 
+bool launch_config_ok(const lpg_tet_energy* h, const LaunchConfig& lc, int vertices_per_tile, int lds_budget) {
+    if (!launch_shape_supported(lc.lanes_per_vertex, lc.waves, h->setup.wide) || vertices_per_tile <= 0 ||
+        vertices_per_tile > lc.waves * (64 / lc.lanes_per_vertex) || lds_budget <= 0)
+        return false;
+    // The budget plus the gradient and energy slots must fit one workgroup's LDS.
+    return assemble_lds_bytes(lc, TileCaps{lds_budget}, false, 0) <= assemble_lds_limit(lc);
+}
+
 void set_plan(lpg_tet_energy* h, const LaunchConfig& lc, int vertices_per_tile, int lds_budget,
               int persistent_grid) {
     TilePlan& p = h->plan;
+    ++h->plan_version;
+    h->graph.reset();
     p.lc = lc;
     p.vertices_per_tile = vertices_per_tile;
     p.lds_budget = lds_budget;
@@ -158,8 +180,10 @@ void ensure_device(lpg_tet_energy* h) {
         }
     upload(h->d_inc, words.data(), words.size() * sizeof(uint32_t));
     h->d_scalars.ensure(64);
-    if (h->seg_vbeg.empty())
+    if (h->seg_vbeg.empty()) {
         h->seg_vbeg = {0, s.num_vertices};
+        h->seg_tbeg = {0, s.num_tets};
+    }
     build_energy_chunks(h->seg_vbeg, h->chunk_begin, h->seg_chunk);
     upload(h->d_chunk_begin, h->chunk_begin.data(), h->chunk_begin.size() * sizeof(int32_t));
     upload(h->d_seg_chunk, h->seg_chunk.data(), h->seg_chunk.size() * sizeof(int32_t));
@@ -167,11 +191,12 @@ void ensure_device(lpg_tet_energy* h) {
     h->d_vertex_E.ensure(std::max<size_t>(1, s.num_vertices) * sizeof(double));
     if (h->plan.tile_vbeg.empty()) {
         // Default launch: 4 lanes per vertex, one wave per workgroup, 16 vertices per tile, segments up to
-        // 4096 doubles (32 KiB) staged in LDS, one workgroup per tile.
+        // 4096 doubles (32 KiB) staged in LDS; one workgroup per tile for a single mesh, a persistent grid of
+        // 2048 workgroups walking the tiles of a batch.
         LaunchConfig lc;
         lc.lanes_per_vertex = 4;
         lc.waves = 1;
-        set_plan(h, lc, 16, 4096, 0);
+        set_plan(h, lc, 16, 4096, h->seg_vbeg.size() > 2 ? 2048 : 0);
     }
     h->device_ready = true;
 }
@@ -280,13 +305,74 @@ void collect_timing(lpg_tet_energy* h) {
 
 namespace {
 
+double read_back(lpg_tet_energy* h, bool want_counter) {
+    double value;
+    unsigned status, counter;
+    std::memcpy(&value, h->pinned, 8);
+    std::memcpy(&status, static_cast<char*>(h->pinned) + 8, 4);
+    std::memcpy(&counter, static_cast<char*>(h->pinned) + 12, 4);
+    if (want_counter)
+        h->last_indefinite = static_cast<int32_t>(counter);
+    raise_status(status);
+    return value;
+}
+
+} // namespace
+
+double run_eval(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g, double* H,
+                double* energies, hipStream_t stream) {
+    ensure_device(h);
+    GraphCache& gc = h->graph;
+    if (gc.enabled && !h->timing) {
+        GraphKey k;
+        k.x = x;
+        k.theta = theta;
+        k.g = want_g ? g : nullptr;
+        k.H = kind >= 0 ? H : nullptr;
+        k.energies = energies;
+        k.want_g = want_g;
+        k.kind = kind;
+        k.plan_version = h->plan_version;
+        if (!(gc.exec && gc.key == k) && gc.pending == k) {
+            // The call repeats: capture the sequence (kernels and the status read-back) on the handle's stream.
+            // Allocations and one-time attribute calls already happened in the eager call before.
+            gc.reset();
+            hip_check(hipStreamBeginCapture(h->own_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try {
+                enqueue_eval(h, x, theta, want_g, kind, g, H, energies, h->own_stream);
+                hip_check(hipMemcpyAsync(h->pinned, h->d_scalars.ptr, 16, hipMemcpyDeviceToHost, h->own_stream),
+                          "read-back");
+            } catch (...) {
+                hipGraph_t broken = nullptr;
+                (void)hipStreamEndCapture(h->own_stream, &broken);
+                if (broken)
+                    (void)hipGraphDestroy(broken);
+                throw;
+            }
+            hip_check(hipStreamEndCapture(h->own_stream, &gc.graph), "hipStreamEndCapture");
+            hip_check(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+            gc.key = k;
+        }
+        if (gc.exec && gc.key == k) {
+            hip_check(hipGraphLaunch(gc.exec, stream), "hipGraphLaunch");
+            hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            return read_back(h, kind == 1);
+        }
+        gc.pending = k;
+    }
+    enqueue_eval(h, x, theta, want_g, kind, g, H, energies, stream);
+    const double value = finish(h, stream, kind == 1);
+    collect_timing(h);
+    return value;
+}
+
+namespace {
+
 void eval_device(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* value,
                  double* g, double* H, hipStream_t stream) {
     if (!value)
         fail(kInvalid, "lpg_tet_energy_eval: value pointer is null");
-    enqueue_eval(h, x, theta, want_g, kind, g, H, nullptr, stream);
-    *value = finish(h, stream, kind == 1);
-    collect_timing(h);
+    *value = run_eval(h, x, theta, want_g, kind, g, H, nullptr, stream);
 }
 
 template <class Launch>
@@ -459,15 +545,11 @@ lpg_status lpg_tet_energy_set_launch(lpg_tet_energy* h, int32_t lanes_per_vertex
         if (!h)
             fail(lpg::kInvalid, "null handle");
         lpg::BusyScope scope(h->busy);
-        if (!lpg::launch_shape_supported(lanes_per_vertex, waves, h->setup.wide) || vertices_per_tile <= 0 ||
-            vertices_per_tile > waves * (64 / lanes_per_vertex) || lds_budget_doubles <= 0)
-            fail(lpg::kInvalid, "lpg_tet_energy_set_launch: unsupported configuration");
         lpg::LaunchConfig lc;
         lc.lanes_per_vertex = lanes_per_vertex;
         lc.waves = waves;
-        // The budget plus the gradient and energy slots must fit one workgroup's LDS.
-        if (lpg::assemble_lds_bytes(lc, lpg::TileCaps{lds_budget_doubles}, false, 0) > lpg::assemble_lds_limit(lc))
-            fail(lpg::kInvalid, "lpg_tet_energy_set_launch: LDS budget exceeds the workgroup's LDS");
+        if (!lpg::launch_config_ok(h, lc, vertices_per_tile, lds_budget_doubles))
+            fail(lpg::kInvalid, "lpg_tet_energy_set_launch: unsupported configuration");
         lpg::ensure_device(h);
         lpg::set_plan(h, lc, vertices_per_tile, lds_budget_doubles, 0);
     });

@@ -40,6 +40,31 @@ struct TilePlan {
     }
 };
 
+// A captured evaluation sequence (hipGraph) and the call it was captured for.  Kernel nodes hold the buffer
+// pointers and the launch plan, so the graph is replayed only for the same pointers, kinds and plan version.
+struct GraphKey {
+    const void* x = nullptr;
+    const void* theta = nullptr;
+    void* g = nullptr;
+    void* H = nullptr;
+    void* energies = nullptr;
+    int want_g = 0;
+    int kind = -2;
+    int plan_version = -1;
+    bool operator==(const GraphKey& o) const {
+        return x == o.x && theta == o.theta && g == o.g && H == o.H && energies == o.energies &&
+               want_g == o.want_g && kind == o.kind && plan_version == o.plan_version;
+    }
+};
+struct GraphCache {
+    bool enabled = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    GraphKey key;     // what `exec` was captured for
+    GraphKey pending; // the last eager call: a repeat of it is captured
+    void reset();
+};
+
 // Overlapping calls on one handle are rejected, as MeshEnergy's BusyScope does (mesh_energy.cpp:49).
 struct BusyScope {
     std::atomic<bool>& flag;
@@ -57,6 +82,7 @@ struct lpg_tet_energy {
     // Vertex ranges of the meshes whose energies are reported separately: {0, V} for a plain energy, the mesh
     // starts plus V for a batch.  Tiles never cross these boundaries.
     std::vector<int32_t> seg_vbeg;
+    std::vector<int32_t> seg_tbeg; // tet offsets of the meshes of a batch ({0, T} for a plain energy)
     std::vector<int32_t> chunk_begin, seg_chunk; // energy reduction chunks (build_energy_chunks)
     int device = 0;
     bool device_ready = false;
@@ -67,6 +93,8 @@ struct lpg_tet_energy {
     lpg::DeviceBuffer d_proj_slot, d_proj_list, d_proj_H;
     lpg::DeviceBuffer d_x, d_g, d_H; // staging for the host-pointer API
     lpg::TilePlan plan;
+    int plan_version = 0;
+    lpg::GraphCache graph;
     std::atomic<bool> busy{false};
     int32_t last_indefinite = 0;
     void* pinned = nullptr; // energy + status + counter read-back
@@ -92,6 +120,8 @@ namespace lpg {
 
 // Uploads the mesh and creates the stream and read-back buffer on first use.
 void ensure_device(lpg_tet_energy* h);
+// Launch shapes accepted by lpg_tet_energy_set_launch (and tried by the autotuner).
+bool launch_config_ok(const lpg_tet_energy* h, const LaunchConfig& lc, int vertices_per_tile, int lds_budget);
 // (Re)builds the tiles for a launch configuration.
 void set_plan(lpg_tet_energy* h, const LaunchConfig& lc, int vertices_per_tile, int lds_budget,
               int persistent_grid);
@@ -101,6 +131,12 @@ void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int w
                   double* energies, hipStream_t stream);
 // Enqueues the read-back of status, counter and the first energy, synchronises, maps the status to an error.
 double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter);
+// One evaluation (eager, or replayed from the handle's captured graph when graph mode is on and the call repeats);
+// returns the first energy (the plain energy; per-mesh energies go to `energies` when given).
+double run_eval(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g, double* H,
+                double* energies, hipStream_t stream);
+// Accumulates the per-phase event times of the last eager evaluation (timing mode).
+void collect_timing(lpg_tet_energy* h);
 // The status mapping alone (status word as read back).
 void raise_status(unsigned status);
 

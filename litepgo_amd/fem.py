@@ -172,6 +172,19 @@ class VolumetricFemEnergy:
         _native.check(_native.lib().lpg_tet_energy_kernel_times(self._h, _ptr(ms), _ptr(n)), "kernel_times")
         return ms, int(n[0])
 
+    def set_graph(self, enable=True):
+        """Graph mode: a repeated device evaluation (same buffers, kind and plan) replays a captured hipGraph."""
+        _native.check(_native.lib().lpg_tet_energy_set_graph(self._h, int(bool(enable))), "set_graph")
+
+    def autotune(self, x, hessian="projected", repeats=3):
+        """Times the candidate launch shapes on this mesh (x: HIP tensor) and keeps the fastest; returns
+        ((lanes, waves, vertices per tile, LDS budget, persistent grid), assembly ms)."""
+        chosen = np.zeros(5, dtype=np.int32)
+        best = np.zeros(1)
+        _native.check(_native.lib().lpg_tet_energy_autotune(self._h, _tptr(x.contiguous()), kind_code(hessian),
+                                                            int(repeats), _ptr(chosen), _ptr(best)), "autotune")
+        return tuple(int(v) for v in chosen), float(best[0])
+
     @property
     def last_indefinite_count(self) -> int:
         return _native.lib().lpg_tet_energy_last_indefinite_count(self._h)

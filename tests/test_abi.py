@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_headers_compile_as_c_and_cpp(tmp_path):
     src = tmp_path / "use.c"
-    src.write_text('#include "litepgo_fem.h"\n#include "litepgo_fem_ext.h"\nint main(void) { return LPG_OK; }\n')
+    src.write_text('#include "litepgo_fem.h"\n#include "litepgo_fem_ext.h"\n#include "litepgo_fem_batch.h"\nint main(void) { return LPG_OK; }\n')
     for compiler, name in (("gcc", "use.c"), ("g++", "use.cpp")):
         target = tmp_path / name
         target.write_text(src.read_text())
