@@ -7,10 +7,12 @@ gradient and the PSD-projected Hessian assembled into the CSR value array, with 
 `value` is tets/s of the whole step (every kernel, the status/energy read-back and the host synchronisation).
 
 Also reported: the assembly kernel against the HBM roofline (HIP events around the kernel on its stream) and its
-FP64 rate against the FP64 vector peak, the per-phase times, the same step with the Exact Hessian, and the oracle
-(C++ CPU restatement of the reference algorithm) timed on this host's cores.
+FP64 rate against the FP64 vector peak, the per-phase times, the same step with the Exact Hessian, the 256-mesh
+batched evaluation (one autotuned persistent launch sequence replayed as a hipGraph) against a per-mesh launch
+loop, the 32^3 differentiation workload's backward time against its forward time, and the oracle (C++ CPU
+restatement of the reference algorithm) timed on this host's cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched]
 """
 from __future__ import annotations
 
@@ -80,6 +82,103 @@ def time_steps(dev, step, kind, steps, warmup):
     return elapsed / steps * 1e3, [m / evals for m in ms]
 
 
+def bench_batched(steps, warmup):
+    """256 uneven meshes: one batched evaluation (autotuned launch, captured hipGraph) against a loop that
+    evaluates each mesh with its own handle, E+g+H(Projected)."""
+    import numpy as np
+    import torch
+
+    from litepgo_amd import BatchedFemEnergy, VolumetricFemEnergy, workloads
+
+    meshes = workloads.batched(256)
+    T = sum(w.tets.shape[0] for w in meshes)
+    batch = BatchedFemEnergy(meshes)
+    x = torch.from_numpy(np.concatenate([w.u for w in meshes])).cuda()
+    shape, tuned_ms = batch.autotune(x, "projected", repeats=3)
+    batch.set_graph(True)
+    E = torch.empty(batch.num_meshes, dtype=torch.float64, device="cuda")
+    g = torch.empty(batch.num_dofs, dtype=torch.float64, device="cuda")
+    H = torch.empty(batch.nnz, dtype=torch.float64, device="cuda")
+    for _ in range(warmup):
+        batch.eval_device(x, True, "projected", out_E=E, out_g=g, out_H=H)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.eval_device(x, True, "projected", out_E=E, out_g=g, out_H=H)
+    torch.cuda.synchronize()
+    batch_ms = (time.perf_counter() - t0) / steps * 1e3
+
+    plains = [VolumetricFemEnergy(w.num_vertices, w.tets, w.rest, w.theta) for w in meshes]
+    xs = [torch.from_numpy(w.u).cuda() for w in meshes]
+    gs = [torch.empty(p.num_dofs, dtype=torch.float64, device="cuda") for p in plains]
+    Hs = [torch.empty(p.nnz, dtype=torch.float64, device="cuda") for p in plains]
+
+    def loop():
+        for p, xm, gm, Hm in zip(plains, xs, gs, Hs):
+            p.eval_device(xm, True, "projected", out_g=gm, out_H=Hm)
+
+    for _ in range(warmup):
+        loop()
+    torch.cuda.synchronize()
+    loop_steps = max(2, steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(loop_steps):
+        loop()
+    torch.cuda.synchronize()
+    loop_ms = (time.perf_counter() - t0) / loop_steps * 1e3
+    return {"meshes": len(meshes), "tets": T, "ms_per_step": batch_ms, "tets_per_s": T / (batch_ms * 1e-3),
+            "per_mesh_loop_ms": loop_ms, "per_mesh_loop_tets_per_s": T / (loop_ms * 1e-3),
+            "speedup_vs_per_mesh_loop": loop_ms / batch_ms,
+            "launch": {"lanes_per_vertex": shape[0], "waves": shape[1], "vertices_per_tile": shape[2],
+                       "lds_budget": shape[3], "persistent_grid": shape[4], "graph": True}}
+
+
+def bench_differentiation(steps, warmup):
+    """32^3 differentiation workload (per-tet log-normal E, uniform nu).  Forward: theta = lame(E, nu), (E, f) by
+    one assembly, and the objective L = E + w . f (w fixed, random).  Backward: dL/dx, dL/dE_young, dL/dnu by
+    autograd (one fused library call for s f + H w and s dE/dtheta + (df/dtheta)^T w, plus the (lambda, mu)
+    chain rule).  Wall-clock medians, the host synchronised with the device on both sides of each part."""
+    import numpy as np
+    import torch
+
+    from litepgo_amd import VolumetricFemEnergy, workloads
+    from litepgo_amd import torch as lt
+
+    w, E_young, nu = workloads.differentiate(seed=0, n=32)
+    pot = VolumetricFemEnergy(w.num_vertices, w.tets, w.rest, w.theta)
+    x = torch.from_numpy(w.u).cuda().requires_grad_(True)
+    Et = torch.from_numpy(E_young).cuda().requires_grad_(True)
+    nut = torch.from_numpy(nu).cuda().requires_grad_(True)
+    wt = torch.from_numpy(np.random.default_rng(1).normal(size=pot.num_dofs)).cuda()
+
+    def forward():
+        E, f = lt.energy_and_force(pot, x, lt.lame(Et, nut))
+        return E + torch.dot(wt, f)
+
+    def backward(objective):
+        return torch.autograd.grad(objective, (x, Et, nut))
+
+    for _ in range(warmup):
+        backward(forward())
+    fwd, bwd = [], []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        objective = forward()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        backward(objective)
+        torch.cuda.synchronize()
+        fwd.append(t1 - t0)
+        bwd.append(time.perf_counter() - t1)
+    f_ms, b_ms = statistics.median(fwd) * 1e3, statistics.median(bwd) * 1e3
+    T = pot.num_tets
+    return {"workload": "32^3 Kuhn grid, per-tet E log-normal(ln 1e5, 0.5), nu U[0.3, 0.45], noise +-0.05h; "
+                        "objective E + w.f, gradients in x, E_young, nu",
+            "tets": T, "forward_ms": f_ms, "backward_ms": b_ms, "backward_over_forward": b_ms / f_ms,
+            "forward_tets_per_s": T / (f_ms * 1e-3), "backward_tets_per_s": T / (b_ms * 1e-3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +187,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--exact-only", action="store_true", help="time only the Exact-Hessian step (profiling aid)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-mesh batched comparison")
+    ap.add_argument("--no-differentiation", action="store_true", help="skip the 32^3 forward/backward timing")
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a known-size kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -162,6 +263,10 @@ def main():
         "indefinite_tets": indefinite, "indefinite_fraction": indefinite / T,
         "exact": exact,
     }
+    if not args.no_batched:
+        result["batched"] = bench_batched(args.steps, args.warmup)
+    if not args.no_differentiation:
+        result["differentiation"] = bench_differentiation(args.steps, args.warmup)
     if not args.no_cpu:
         nthreads = min(16, len(os.sched_getaffinity(0)))
         result["cpu_baseline"] = cpu_baseline(w, nthreads, args.cpu_repeats)

@@ -108,6 +108,15 @@ lpg_status lpg_tet_energy_force_jvp_theta_device(lpg_tet_energy* energy, const d
                                                  void* stream);
 lpg_status lpg_tet_energy_hvp_device(lpg_tet_energy* energy, const double* x_dev, const double* theta_dev,
                                      const double* v_dev, double* out_dev, void* stream);
+/* Reverse product of (E, f = dE/dx) in one call, the fused backward of litepgo.torch's energy and response
+ * adapters (src/python/torch.py:139-142 gives upstream * gradient, :154-157 the response VJP):
+ *   gx     = s f + H w                       (num_dofs; H the Exact Hessian),
+ *   gtheta = s dE/dtheta + (df/dtheta)^T w   (num_tets x 2),
+ * with s = *gE_dev (a device scalar; 0 when gE_dev is NULL) and w = w_dev (a zero vector when NULL).  gx_dev or
+ * gtheta_dev may be NULL to skip that output.  One status read-back for both outputs. */
+lpg_status lpg_tet_energy_vjp_device(lpg_tet_energy* energy, const double* x_dev, const double* theta_dev,
+                                     const double* gE_dev, const double* w_dev, double* gx_dev, double* gtheta_dev,
+                                     void* stream);
 
 #ifdef __cplusplus
 }

@@ -30,6 +30,15 @@ lpg_status lpg_tet_energy_kernel_times(const lpg_tet_energy* energy, double* ms,
 /* Largest vertex degree (distinct neighbours incl. itself) of the mesh. */
 int32_t lpg_tet_energy_max_degree(const lpg_tet_energy* energy);
 
+/* Per-element Lame parameters from Young's modulus and Poisson ratio (device arrays of n entries), enqueued on
+ * `stream` with no synchronisation: theta_dev (n x 2) <- (E nu / ((1 + nu)(1 - 2 nu)), E / (2 (1 + nu))).
+ * lame_vjp gives the reverse product (gE, gnu) = (g_lambda, g_mu) d(lambda, mu)/d(E, nu) for the cotangent
+ * gtheta_dev (n x 2); gE_dev or gnu_dev may be NULL.  No counterpart in the reference (its materials take
+ * (lambda, mu) directly); used by the torch bridge to differentiate with respect to E and nu fields. */
+lpg_status lpg_lame_device(const double* E_dev, const double* nu_dev, int64_t n, double* theta_dev, void* stream);
+lpg_status lpg_lame_vjp_device(const double* E_dev, const double* nu_dev, const double* gtheta_dev, int64_t n,
+                               double* gE_dev, double* gnu_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

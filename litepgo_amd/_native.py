@@ -28,11 +28,14 @@ SIGNATURES = {
     "lpg_tet_energy_force_vjp_theta_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "lpg_tet_energy_force_jvp_theta_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "lpg_tet_energy_hvp_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "lpg_tet_energy_vjp_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "lpg_tet_energy_set_launch": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32]),
     "lpg_tet_energy_get_launch": (ctypes.c_int, [_P, _P]),
     "lpg_tet_energy_max_degree": (_I32, [_P]),
     "lpg_tet_energy_set_timing": (ctypes.c_int, [_P, _I32]),
     "lpg_tet_energy_kernel_times": (ctypes.c_int, [_P, _P, _P]),
+    "lpg_lame_device": (ctypes.c_int, [_P, _P, ctypes.c_int64, _P, _P]),
+    "lpg_lame_vjp_device": (ctypes.c_int, [_P, _P, _P, ctypes.c_int64, _P, _P, _P]),
     "lpg_tet_batch_create": (ctypes.c_int, [_I32, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]),
     "lpg_tet_batch_num_meshes": (_I32, [_P]),
     "lpg_tet_batch_offsets": (ctypes.c_int, [_P, _P, _P, _P]),

@@ -538,7 +538,33 @@ lpg_status lpg_tet_energy_hvp_device(lpg_tet_energy* h, const double* x_dev, con
                                 });
 }
 
+lpg_status lpg_tet_energy_vjp_device(lpg_tet_energy* h, const double* x_dev, const double* theta_dev,
+                                     const double* gE_dev, const double* w_dev, double* gx_dev, double* gtheta_dev,
+                                     void* stream) {
+    return lpg::derivative_call(h, x_dev, theta_dev, stream,
+                                [&](const lpg::DeviceMesh& m, unsigned* st, hipStream_t s) {
+                                    lpg::launch_vjp(m, x_dev, theta_dev, gE_dev, w_dev, gx_dev, gtheta_dev, st, s);
+                                });
+}
+
 // ---- extensions (litepgo_fem_ext.h) ----
+lpg_status lpg_lame_device(const double* E_dev, const double* nu_dev, int64_t n, double* theta_dev, void* stream) {
+    return guarded([&] {
+        if (n < 0 || (n > 0 && (!E_dev || !nu_dev || !theta_dev)))
+            fail(lpg::kInvalid, "lpg_lame_device: invalid argument");
+        lpg::launch_lame(E_dev, nu_dev, n, theta_dev, static_cast<hipStream_t>(stream));
+    });
+}
+
+lpg_status lpg_lame_vjp_device(const double* E_dev, const double* nu_dev, const double* gtheta_dev, int64_t n,
+                               double* gE_dev, double* gnu_dev, void* stream) {
+    return guarded([&] {
+        if (n < 0 || (n > 0 && (!E_dev || !nu_dev || !gtheta_dev)))
+            fail(lpg::kInvalid, "lpg_lame_vjp_device: invalid argument");
+        lpg::launch_lame_vjp(E_dev, nu_dev, gtheta_dev, n, gE_dev, gnu_dev, static_cast<hipStream_t>(stream));
+    });
+}
+
 lpg_status lpg_tet_energy_set_launch(lpg_tet_energy* h, int32_t lanes_per_vertex, int32_t waves,
                                      int32_t vertices_per_tile, int32_t lds_budget_doubles) {
     return guarded([&] {

@@ -171,5 +171,13 @@ void launch_force_jvp_theta(const DeviceMesh& m, const double* x, const double* 
                             double* out, unsigned* status, hipStream_t stream);
 void launch_hvp(const DeviceMesh& m, const double* x, const double* theta, const double* v, double* out,
                 unsigned* status, hipStream_t stream);
+// gx = s f + H w and gtheta = s dE/dtheta + (df/dtheta)^T w with s = *gE (0 if null), w optional; either
+// output may be null.
+void launch_vjp(const DeviceMesh& m, const double* x, const double* theta, const double* gE, const double* w,
+                double* gx, double* gtheta, unsigned* status, hipStream_t stream);
+// theta_e = (lambda, mu)(E_e, nu_e) and its reverse product.
+void launch_lame(const double* E, const double* nu, int64_t n, double* theta, hipStream_t stream);
+void launch_lame_vjp(const double* E, const double* nu, const double* gtheta, int64_t n, double* gE, double* gnu,
+                     hipStream_t stream);
 
 } // namespace lpg

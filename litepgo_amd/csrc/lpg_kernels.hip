@@ -26,6 +26,7 @@
 #include "lpg_internal.hpp"
 #include "lpg_eig9.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 // No implicit FMA contraction: every fused multiply-add below is an explicit fma(), so each value is computed by the
@@ -992,14 +993,21 @@ struct ElemParams {
     const double* theta_snap;
     const double* x;
     const double* theta;
-    const double* w;      // adjoint vector (num_dofs) for the VJP
+    const double* w;      // force cotangent (num_dofs)
+    const double* scale;  // device scalar multiplying dE/dtheta, or null for scale_val
+    double scale_val;
     double* out;          // T x 2
     unsigned* status;
     int num_tets;
 };
 
-// dE/dtheta_e = w (L^2 / 2, (|F|^2 - 3) / 2 - L).
-__global__ __launch_bounds__(256) void k_dE_dtheta(ElemParams p) {
+// Per-tet theta cotangent: out_e = s dE/dtheta_e + (d g_e / d theta_e)^T w, with
+//   dE/dtheta_e = w_e (L^2 / 2, (|F|^2 - 3) / 2 - L)
+//   d g_e[3a+i] / d lambda = w_e L q_a,i,  d / d mu = w_e ((F g_a)_i - q_a,i)
+// (psi and P = mu (F - F^-T) + lambda L F^-T of neo_hookean.cpp:88-101, P linear in theta).
+// HAS_S drops the second term's absence; HAS_W the first's: <true, false> is dE/dtheta (s = 1), <false, true>
+// the force VJP, <true, true> both in one pass.
+template <bool HAS_S, bool HAS_W> __global__ __launch_bounds__(256) void k_theta_vjp(ElemParams p) {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= p.num_tets)
         return;
@@ -1015,59 +1023,38 @@ __global__ __launch_bounds__(256) void k_dE_dtheta(ElemParams p) {
         flags |= kStatusDetF;
     if (!theta_ok(t.lam, t.mu))
         flags |= kStatusBadTheta;
-    double sq = 0.0;
+    double2 r = make_double2(0.0, 0.0);
+    if (HAS_S) {
+        const double s = p.scale ? *p.scale : p.scale_val;
+        double sq = 0.0;
 #pragma unroll
-    for (int q = 0; q < 9; ++q)
-        sq = fma(k.F[q], k.F[q], sq);
-    double2 r;
-    r.x = geo.w * (0.5 * k.L * k.L);
-    r.y = geo.w * (0.5 * (sq - 3.0) - k.L);
-    if (!(isfinite(r.x) && isfinite(r.y)))
-        flags |= kStatusNonFiniteOut;
-    reinterpret_cast<double2*>(p.out)[e] = r;
-    if (flags)
-        atomicOr(p.status, flags);
-}
-
-// Force VJP in theta: d g_e[3a+i] / d lambda = w L q_a,i and d / d mu = w ((F g_a)_i - q_a,i)
-// (dP/dlambda = L F^-T, dP/dmu = F - F^-T; neo_hookean.cpp:88-101).
-__global__ __launch_bounds__(256) void k_force_vjp_theta(ElemParams p) {
-    const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= p.num_tets)
-        return;
-    int ids[4];
-    TetIn t;
-    load_tet(p.Xr, p.conn, p.theta_snap, p.x, p.theta, e, ids, t);
-    Geo geo;
-    rest_geometry(t, geo);
-    Kin k;
-    kinematics(t, geo.Di, k);
-    unsigned flags = 0;
-    if (!kin_ok(k))
-        flags |= kStatusDetF;
-    if (!theta_ok(t.lam, t.mu))
-        flags |= kStatusBadTheta;
-    double G[4][3];
-    shape_gradients(geo.Di, G);
-    double sl = 0.0, sm = 0.0;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const double* wb = p.w + 3 * static_cast<size_t>(ids[b]);
-        const double w0 = wb[0], w1 = wb[1], w2 = wb[2];
-        double q[3], Fg[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            q[i] = dot3(k.a[3 * i], k.a[3 * i + 1], k.a[3 * i + 2], G[b][0], G[b][1], G[b][2]);
-            Fg[i] = dot3(k.F[3 * i], k.F[3 * i + 1], k.F[3 * i + 2], G[b][0], G[b][1], G[b][2]);
-        }
-        const double qw = dot3(q[0], q[1], q[2], w0, w1, w2);
-        const double fw = dot3(Fg[0], Fg[1], Fg[2], w0, w1, w2);
-        sl += qw;
-        sm += fw - qw;
+        for (int q = 0; q < 9; ++q)
+            sq = fma(k.F[q], k.F[q], sq);
+        r.x = s * (geo.w * (0.5 * k.L * k.L));
+        r.y = s * (geo.w * (0.5 * (sq - 3.0) - k.L));
     }
-    double2 r;
-    r.x = geo.w * k.L * sl;
-    r.y = geo.w * sm;
+    if (HAS_W) {
+        double G[4][3];
+        shape_gradients(geo.Di, G);
+        double sl = 0.0, sm = 0.0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const double* wb = p.w + 3 * static_cast<size_t>(ids[b]);
+            const double w0 = wb[0], w1 = wb[1], w2 = wb[2];
+            double q[3], Fg[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                q[i] = dot3(k.a[3 * i], k.a[3 * i + 1], k.a[3 * i + 2], G[b][0], G[b][1], G[b][2]);
+                Fg[i] = dot3(k.F[3 * i], k.F[3 * i + 1], k.F[3 * i + 2], G[b][0], G[b][1], G[b][2]);
+            }
+            const double qw = dot3(q[0], q[1], q[2], w0, w1, w2);
+            const double fw = dot3(Fg[0], Fg[1], Fg[2], w0, w1, w2);
+            sl += qw;
+            sm += fw - qw;
+        }
+        r.x += geo.w * k.L * sl;
+        r.y += geo.w * sm;
+    }
     if (!(isfinite(r.x) && isfinite(r.y)))
         flags |= kStatusNonFiniteOut;
     reinterpret_cast<double2*>(p.out)[e] = r;
@@ -1075,7 +1062,12 @@ __global__ __launch_bounds__(256) void k_force_vjp_theta(ElemParams p) {
         atomicOr(p.status, flags);
 }
 
-// Vertex-gather products: MODE 0 = Exact H v, MODE 1 = sum_e (d g_e / d theta_e) dtheta_e.
+// Vertex-gather products over each vertex's incidence list:
+//   MODE 0: (H v)_v                                (Exact Hessian-vector product)
+//   MODE 1: sum_e (d g_e / d theta_e) dtheta_e     (force JVP in theta; v = dtheta, T x 2)
+//   MODE 2: s f_v + (H v)_v                        (s = *scale or 0, f = dE/dx; v may be null)
+// kGatherLanes lanes share a vertex and stride over its incidences; their partial sums are combined by a fixed
+// xor butterfly, so every lane of the group ends with the same, launch-invariant value.
 struct GatherParams {
     const double* Xr;
     const int32_t* conn;
@@ -1085,20 +1077,30 @@ struct GatherParams {
     int inc_stride;
     const double* x;
     const double* theta;
-    const double* v; // MODE 0: vector (num_dofs); MODE 1: dtheta (T x 2)
+    const double* v;
+    const double* scale;
     double* out;     // num_dofs
     unsigned* status;
     int num_vertices;
 };
 
-template <int MODE> __global__ __launch_bounds__(256) void k_vertex_gather(GatherParams p) {
-    const int v = blockIdx.x * 256 + threadIdx.x;
+constexpr int kGatherLanes = 8;
+constexpr int kGatherThreads = 256;
+
+template <int MODE> __global__ __launch_bounds__(kGatherThreads) void k_vertex_gather(GatherParams p) {
+    static_assert(kGatherThreads % kGatherLanes == 0 && kWave % kGatherLanes == 0, "groups stay inside a wave");
+    const int gid = blockIdx.x * kGatherThreads + threadIdx.x;
+    const int v = gid / kGatherLanes;
+    const int sub = gid % kGatherLanes;
+    // A group is either wholly past the end or wholly active, so the shuffles below read active lanes only.
     if (v >= p.num_vertices)
         return;
     unsigned flags = 0;
     double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    const bool has_v = MODE != 2 || p.v != nullptr;
+    const double s = (MODE == 2 && p.scale) ? *p.scale : 0.0;
     const int kb = p.inc_ptr[v], ke = p.inc_ptr[v + 1];
-    for (int kk = kb; kk < ke; ++kk) {
+    for (int kk = kb + sub; kk < ke; kk += kGatherLanes) {
         const int e = *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(p.inc_tet) +
                                                          static_cast<size_t>(kk) * p.inc_stride);
         int ids[4];
@@ -1121,7 +1123,27 @@ template <int MODE> __global__ __launch_bounds__(256) void k_vertex_gather(Gathe
 #pragma unroll
         for (int i = 0; i < 3; ++i)
             qa[i] = dot3(k.a[3 * i], k.a[3 * i + 1], k.a[3 * i + 2], ga[0], ga[1], ga[2]);
-        if (MODE == 0) {
+        if (MODE == 1 || MODE == 2) {
+            double Fg[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                Fg[i] = dot3(k.F[3 * i], k.F[3 * i + 1], k.F[3 * i + 2], ga[0], ga[1], ga[2]);
+            if (MODE == 1) {
+                const double2 dt = reinterpret_cast<const double2*>(p.v)[e];
+                const double dl = dt.x * k.L;
+                y0 = fma(geo.w, fma(dl, qa[0], dt.y * (Fg[0] - qa[0])), y0);
+                y1 = fma(geo.w, fma(dl, qa[1], dt.y * (Fg[1] - qa[1])), y1);
+                y2 = fma(geo.w, fma(dl, qa[2], dt.y * (Fg[2] - qa[2])), y2);
+            } else {
+                // f_a = w (mu F g_a + (lambda L - mu) q_a)
+                const double pr = fma(t.lam, k.L, -t.mu);
+                const double sw = s * geo.w;
+                y0 = fma(sw, fma(t.mu, Fg[0], pr * qa[0]), y0);
+                y1 = fma(sw, fma(t.mu, Fg[1], pr * qa[1]), y1);
+                y2 = fma(sw, fma(t.mu, Fg[2], pr * qa[2]), y2);
+            }
+        }
+        if ((MODE == 0 || MODE == 2) && has_v) {
             const double pr = fma(t.lam, k.L, -t.mu);
             // sum_b H_ab v_b = w sum_b [mu (g_a.g_b) v_b + lambda q_a (q_b.v_b) - p q_b (q_a.v_b)]
             double sv[3] = {0.0, 0.0, 0.0}, sq[3] = {0.0, 0.0, 0.0};
@@ -1147,26 +1169,54 @@ template <int MODE> __global__ __launch_bounds__(256) void k_vertex_gather(Gathe
             y0 = fma(geo.w, fma(t.mu, sv[0], fma(t.lam * qa[0], sqv, -(pr * sq[0]))), y0);
             y1 = fma(geo.w, fma(t.mu, sv[1], fma(t.lam * qa[1], sqv, -(pr * sq[1]))), y1);
             y2 = fma(geo.w, fma(t.mu, sv[2], fma(t.lam * qa[2], sqv, -(pr * sq[2]))), y2);
-        } else {
-            const double2 dt = reinterpret_cast<const double2*>(p.v)[e];
-            double Fg[3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-                Fg[i] = dot3(k.F[3 * i], k.F[3 * i + 1], k.F[3 * i + 2], ga[0], ga[1], ga[2]);
-            const double dl = dt.x * k.L;
-            y0 = fma(geo.w, fma(dl, qa[0], dt.y * (Fg[0] - qa[0])), y0);
-            y1 = fma(geo.w, fma(dl, qa[1], dt.y * (Fg[1] - qa[1])), y1);
-            y2 = fma(geo.w, fma(dl, qa[2], dt.y * (Fg[2] - qa[2])), y2);
         }
     }
-    double* o = p.out + 3 * static_cast<size_t>(v);
-    o[0] = y0;
-    o[1] = y1;
-    o[2] = y2;
-    if (!(isfinite(y0) && isfinite(y1) && isfinite(y2)))
-        flags |= kStatusNonFiniteOut;
+#pragma unroll
+    for (int m = kGatherLanes / 2; m > 0; m >>= 1) {
+        y0 += __shfl_xor(y0, m, kGatherLanes);
+        y1 += __shfl_xor(y1, m, kGatherLanes);
+        y2 += __shfl_xor(y2, m, kGatherLanes);
+    }
+    if (sub == 0) {
+        double* o = p.out + 3 * static_cast<size_t>(v);
+        o[0] = y0;
+        o[1] = y1;
+        o[2] = y2;
+        if (!(isfinite(y0) && isfinite(y1) && isfinite(y2)))
+            flags |= kStatusNonFiniteOut;
+    }
     if (flags)
         atomicOr(p.status, flags);
+}
+
+// Lame parameters from Young's modulus and Poisson ratio, per element:
+//   lambda = E nu / ((1 + nu)(1 - 2 nu)),  mu = E / (2 (1 + nu)),
+// and the reverse product (g_E, g_nu) = (g_lambda, g_mu) d(lambda, mu)/d(E, nu) with
+//   dlambda/dE = nu / D, dlambda/dnu = E (1 + 2 nu^2) / D^2, dmu/dE = 1 / (2 (1 + nu)), dmu/dnu = -E / (2 (1 + nu)^2),
+// D = (1 + nu)(1 - 2 nu).
+__global__ __launch_bounds__(256) void k_lame(const double* __restrict__ E, const double* __restrict__ nu, long long n,
+                                              double2* __restrict__ theta) {
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += 256ll * gridDim.x) {
+        const double e = E[i], v = nu[i];
+        const double op = 1.0 + v;
+        theta[i] = make_double2(e * v / (op * (1.0 - 2.0 * v)), e / (2.0 * op));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_lame_vjp(const double* __restrict__ E, const double* __restrict__ nu,
+                                                  const double2* __restrict__ gtheta, long long n,
+                                                  double* __restrict__ gE, double* __restrict__ gnu) {
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += 256ll * gridDim.x) {
+        const double e = E[i], v = nu[i];
+        const double2 g = gtheta[i];
+        const double op = 1.0 + v;
+        const double D = op * (1.0 - 2.0 * v);
+        const double inv_D = 1.0 / D, inv_op = 1.0 / op;
+        if (gE)
+            gE[i] = g.x * (v * inv_D) + g.y * (0.5 * inv_op);
+        if (gnu)
+            gnu[i] = g.x * (e * fma(2.0 * v, v, 1.0) * inv_D * inv_D) - g.y * (0.5 * e * inv_op * inv_op);
+    }
 }
 
 void check_launch(const char* what) {
@@ -1333,26 +1383,34 @@ void launch_project_jacobi(const DeviceMesh& m, const EvalArgs& a, hipStream_t s
     check_launch("k_project_jacobi");
 }
 
-void launch_dE_dtheta(const DeviceMesh& m, const double* x, const double* theta, double* out, unsigned* status,
-                      hipStream_t stream) {
+static void launch_theta_vjp(const DeviceMesh& m, const double* x, const double* theta, const double* scale,
+                             double scale_val, bool has_s, const double* w, double* out, unsigned* status,
+                             hipStream_t stream) {
     if (m.num_tets == 0)
         return;
-    ElemParams p{m.Xr, m.conn, m.theta, x, theta, nullptr, out, status, m.num_tets};
-    hipLaunchKernelGGL(k_dE_dtheta, dim3((m.num_tets + 255) / 256), dim3(256), 0, stream, p);
-    check_launch("k_dE_dtheta");
+    ElemParams p{m.Xr, m.conn, m.theta, x, theta, w, scale, scale_val, out, status, m.num_tets};
+    const dim3 grid((m.num_tets + 255) / 256), block(256);
+    if (has_s && w)
+        hipLaunchKernelGGL((k_theta_vjp<true, true>), grid, block, 0, stream, p);
+    else if (has_s)
+        hipLaunchKernelGGL((k_theta_vjp<true, false>), grid, block, 0, stream, p);
+    else
+        hipLaunchKernelGGL((k_theta_vjp<false, true>), grid, block, 0, stream, p);
+    check_launch("k_theta_vjp");
+}
+
+void launch_dE_dtheta(const DeviceMesh& m, const double* x, const double* theta, double* out, unsigned* status,
+                      hipStream_t stream) {
+    launch_theta_vjp(m, x, theta, nullptr, 1.0, true, nullptr, out, status, stream);
 }
 
 void launch_force_vjp_theta(const DeviceMesh& m, const double* x, const double* theta, const double* w,
                             double* out, unsigned* status, hipStream_t stream) {
-    if (m.num_tets == 0)
-        return;
-    ElemParams p{m.Xr, m.conn, m.theta, x, theta, w, out, status, m.num_tets};
-    hipLaunchKernelGGL(k_force_vjp_theta, dim3((m.num_tets + 255) / 256), dim3(256), 0, stream, p);
-    check_launch("k_force_vjp_theta");
+    launch_theta_vjp(m, x, theta, nullptr, 0.0, false, w, out, status, stream);
 }
 
 static void launch_gather(int mode, const DeviceMesh& m, const double* x, const double* theta, const double* v,
-                          double* out, unsigned* status, hipStream_t stream) {
+                          const double* scale, double* out, unsigned* status, hipStream_t stream) {
     if (m.num_vertices == 0)
         return;
     GatherParams p{m.Xr,
@@ -1364,25 +1422,58 @@ static void launch_gather(int mode, const DeviceMesh& m, const double* x, const 
                    x,
                    theta,
                    v,
+                   scale,
                    out,
                    status,
                    m.num_vertices};
-    const dim3 grid((m.num_vertices + 255) / 256), block(256);
+    const long long threads = static_cast<long long>(m.num_vertices) * kGatherLanes;
+    const dim3 grid(static_cast<unsigned>((threads + kGatherThreads - 1) / kGatherThreads)), block(kGatherThreads);
     if (mode == 0)
         hipLaunchKernelGGL(k_vertex_gather<0>, grid, block, 0, stream, p);
-    else
+    else if (mode == 1)
         hipLaunchKernelGGL(k_vertex_gather<1>, grid, block, 0, stream, p);
+    else
+        hipLaunchKernelGGL(k_vertex_gather<2>, grid, block, 0, stream, p);
     check_launch("k_vertex_gather");
 }
 
 void launch_force_jvp_theta(const DeviceMesh& m, const double* x, const double* theta, const double* dtheta,
                             double* out, unsigned* status, hipStream_t stream) {
-    launch_gather(1, m, x, theta, dtheta, out, status, stream);
+    launch_gather(1, m, x, theta, dtheta, nullptr, out, status, stream);
 }
 
 void launch_hvp(const DeviceMesh& m, const double* x, const double* theta, const double* v, double* out,
                 unsigned* status, hipStream_t stream) {
-    launch_gather(0, m, x, theta, v, out, status, stream);
+    launch_gather(0, m, x, theta, v, nullptr, out, status, stream);
+}
+
+void launch_vjp(const DeviceMesh& m, const double* x, const double* theta, const double* gE, const double* w,
+                double* gx, double* gtheta, unsigned* status, hipStream_t stream) {
+    if (gx)
+        launch_gather(2, m, x, theta, w, gE, gx, status, stream);
+    if (gtheta) // with neither cotangent, the kernel writes s dE/dtheta with s = 0
+        launch_theta_vjp(m, x, theta, gE, 0.0, gE != nullptr || !w, w, gtheta, status, stream);
+}
+
+static unsigned elementwise_blocks(int64_t n) {
+    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)));
+}
+
+void launch_lame(const double* E, const double* nu, int64_t n, double* theta, hipStream_t stream) {
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL(k_lame, dim3(elementwise_blocks(n)), dim3(256), 0, stream, E, nu, static_cast<long long>(n),
+                       reinterpret_cast<double2*>(theta));
+    check_launch("k_lame");
+}
+
+void launch_lame_vjp(const double* E, const double* nu, const double* gtheta, int64_t n, double* gE, double* gnu,
+                     hipStream_t stream) {
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL(k_lame_vjp, dim3(elementwise_blocks(n)), dim3(256), 0, stream, E, nu,
+                       reinterpret_cast<const double2*>(gtheta), static_cast<long long>(n), gE, gnu);
+    check_launch("k_lame_vjp");
 }
 
 } // namespace lpg

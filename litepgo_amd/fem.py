@@ -260,6 +260,44 @@ class VolumetricFemEnergy:
                       "eval_device")
         return float(value[0]), g, H
 
+    def eval_device_energy(self, x, gradient=False, hessian=None, theta=None, out_E=None, out_g=None, out_H=None):
+        """As eval_device, but the energy is written to a one-entry device tensor (returned first) instead of
+        being returned as a Python float; the call still synchronises once to check the status."""
+        import torch
+
+        self._check_tensor(x, self.num_dofs, "x")
+        if theta is not None:
+            self._check_tensor(theta, 2 * self.num_tets, "theta")
+        kind = kind_code(hessian)
+        E = out_E if out_E is not None else torch.empty(1, dtype=torch.float64, device=x.device)
+        self._check_tensor(E, 1, "out_E")
+        g = None
+        if gradient:
+            g = out_g if out_g is not None else torch.empty(self.num_dofs, dtype=torch.float64, device=x.device)
+        H = None
+        if kind >= 0:
+            H = out_H if out_H is not None else torch.empty(self.nnz, dtype=torch.float64, device=x.device)
+        _native.check(_native.lib().lpg_tet_batch_eval_device(self._h, _tptr(x), _tptr(theta), int(bool(gradient)),
+                                                              kind, _tptr(E), _tptr(g), _tptr(H), _stream()),
+                      "eval_device")
+        return E, g, H
+
+    def vjp_device(self, x, gE=None, w=None, theta=None, want_x=True, want_theta=True):
+        """(gx, gtheta) = (s f + H w, s dE/dtheta + (df/dtheta)^T w), s the one-entry device tensor gE (0 if
+        None), w a num_dofs device tensor (0 if None); either output is None when not wanted."""
+        import torch
+
+        self._check_tensor(x, self.num_dofs, "x")
+        if gE is not None:
+            self._check_tensor(gE, 1, "gE")
+        if w is not None:
+            self._check_tensor(w, self.num_dofs, "w")
+        gx = torch.empty(self.num_dofs, dtype=torch.float64, device=x.device) if want_x else None
+        gt = torch.empty((self.num_tets, 2), dtype=torch.float64, device=x.device) if want_theta else None
+        _native.check(_native.lib().lpg_tet_energy_vjp_device(self._h, _tptr(x), _tptr(theta), _tptr(gE), _tptr(w),
+                                                              _tptr(gx), _tptr(gt), _stream()), "vjp")
+        return gx, gt
+
     def dE_dtheta_device(self, x, theta=None):
         import torch
 
