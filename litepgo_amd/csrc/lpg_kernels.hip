@@ -404,6 +404,30 @@ __device__ __forceinline__ void incidence_values(const TetIn& t, int a, double o
     }
 }
 
+// Rest and current positions of an incidence's three other vertices and its tet's (lambda, mu).
+struct NbrData {
+    double X[3][3];
+    double u[3][3];
+    double lam, mu;
+};
+
+__device__ __forceinline__ void load_nbr_data(const AssembleParams& p, const int (&id)[3], int tet, NbrData& d) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const double4 X = reinterpret_cast<const double4*>(p.Xr)[id[j]];
+        d.X[j][0] = X.x;
+        d.X[j][1] = X.y;
+        d.X[j][2] = X.z;
+        const double* pu = p.x + 3 * static_cast<size_t>(id[j]);
+        d.u[j][0] = pu[0];
+        d.u[j][1] = pu[1];
+        d.u[j][2] = pu[2];
+    }
+    const double2 th = reinterpret_cast<const double2*>(p.theta ? p.theta : p.theta_snap)[tet];
+    d.lam = th.x;
+    d.mu = th.y;
+}
+
 // One tile: zero the tile's H segment in LDS (GLOBAL_ACC: in HBM, for a segment above the LDS budget), accumulate,
 // write it back coalesced, write the gradient rows and per-vertex energies, OR the status flags.
 // LDS layout in doubles: [gradient: 3 per vertex][energy: 1 per vertex][H segment of the tile].
@@ -449,58 +473,36 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
         if (sub == 0 && !(isfinite(uv0) && isfinite(uv1) && isfinite(uv2)))
             flags |= kStatusNonFiniteX;
         const int kb = p.inc_ptr[v], ke = p.inc_ptr[v + 1];
-        // Software pipeline: the record of the round after next and the neighbour ids of the next round are
-        // loaded while the current round computes, so a round waits for one gather (vertex data) instead of three
-        // dependent loads (record -> neighbour ids -> vertex data).
-        Inc r_cur{}, r_next{};
-        int id_cur[3] = {0, 0, 0};
-        int slot_cur = -1; // projection slot of the current incidence's tet (Projected)
-        {
-            const int k = kb + sub;
-            if (k < ke)
-                r_cur = inc[k];
-            if (k + LPV < ke)
-                r_next = inc[k + LPV];
-            if (k < ke) {
-#pragma unroll
-                for (int j = 1; j < 4; ++j)
-                    id_cur[j - 1] = my_nbr[IncT<WIDE>::cp(r_cur, j)];
-                if (KIND == 1)
-                    slot_cur = p.proj_slot[r_cur.tet];
-            }
-        }
-        // Indefinite incidences met in this pass (Projected): the records and projection slots of up to two per
-        // lane are kept for the second pass; more set `pend_all`, which rescans the lane's incidences.
+        // Software pipeline over the lane's rounds m (incidence k = kb + sub + m LPV), three stages deep: while
+        // round m computes, the vertex data of round m+1, the neighbour ids of round m+2 and the record of round
+        // m+3 are in flight.  Rounds alternate between two vertex-data buffers (the loop body holds two rounds),
+        // so no register copy waits on a load that is still in flight.  Indices past the lane's last incidence
+        // are clamped to it: every load stays in bounds and is branch-free, and a clamped round computes nothing.
         Inc pend_r0{}, pend_r1{};
         int pend_s0 = -1, pend_s1 = -1;
         bool pend_all = false;
-        for (int k0 = kb; k0 < ke; k0 += LPV) {
-            const int k = k0 + sub;
-            const bool active = k < ke;
-            Inc r_after{};
-            int id_next[3] = {0, 0, 0};
-            int slot_next = -1;
-            if (k + 2 * LPV < ke)
-                r_after = inc[k + 2 * LPV];
-            if (k + LPV < ke) {
+        const auto clampk = [&](int k) { return k < ke ? k : ke - 1; };
+        const auto load_ids = [&](const Inc& r, int (&id)[3]) {
 #pragma unroll
-                for (int j = 1; j < 4; ++j)
-                    id_next[j - 1] = my_nbr[IncT<WIDE>::cp(r_next, j)];
-                if (KIND == 1)
-                    slot_next = p.proj_slot[r_next.tet];
-            }
-            const Inc r = r_cur;
+            for (int j = 1; j < 4; ++j)
+                id[j - 1] = my_nbr[IncT<WIDE>::cp(r, j)];
+        };
+        const auto load_data = [&](const Inc& r, const int (&id)[3], NbrData& d, int& slot) {
+            load_nbr_data(p, id, r.tet, d);
+            if (KIND == 1)
+                slot = p.proj_slot[r.tet];
+        };
+        // One round: the incidence's values, the register sums, and the off-diagonal blocks into the segment.
+        // Indefinite incidences (Projected) keep the records and projection slots of up to two per lane for the
+        // second pass; more set `pend_all`, which rescans the lane's incidences.
+        const auto round = [&](const Inc& r, const NbrData& d, int slot, bool active) {
             double off[3][9];
             int cps[3] = {0, 0, 0};
             bool add = false;
             if (active) {
-                int ids[4];
-                ids[0] = v;
 #pragma unroll
-                for (int j = 1; j < 4; ++j) {
+                for (int j = 1; j < 4; ++j)
                     cps[j - 1] = IncT<WIDE>::cp(r, j);
-                    ids[j] = id_cur[j - 1];
-                }
                 TetIn t;
                 t.X[0][0] = Xv.x;
                 t.X[0][1] = Xv.y;
@@ -509,17 +511,14 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
                 t.u[0][1] = uv1;
                 t.u[0][2] = uv2;
 #pragma unroll
-                for (int j = 1; j < 4; ++j) {
-                    const double4 X = reinterpret_cast<const double4*>(p.Xr)[ids[j]];
-                    t.X[j][0] = X.x;
-                    t.X[j][1] = X.y;
-                    t.X[j][2] = X.z;
-                    const double* pu = p.x + 3 * static_cast<size_t>(ids[j]);
-                    t.u[j][0] = pu[0];
-                    t.u[j][1] = pu[1];
-                    t.u[j][2] = pu[2];
-                }
-                load_theta(p.theta_snap, p.theta, r.tet, t);
+                for (int j = 1; j < 4; ++j)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        t.X[j][c] = d.X[j - 1][c];
+                        t.u[j][c] = d.u[j - 1][c];
+                    }
+                t.lam = d.lam;
+                t.mu = d.mu;
                 double gc[3], ec = 0.0, dg[6];
                 incidence_values<KIND>(t, IncT<WIDE>::slot(r), off, dg, gc, ec, flags);
                 gs0 += gc[0];
@@ -527,13 +526,13 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
                 gs2 += gc[2];
                 es += ec;
                 // Indefinite tets (Projected) are added by the second pass below.
-                add = KIND == 0 || (KIND == 1 && slot_cur < 0);
-                if (KIND == 1 && slot_cur >= 0) {
+                add = KIND == 0 || (KIND == 1 && slot < 0);
+                if (KIND == 1 && slot >= 0) {
                     if (pend_s0 < 0) {
-                        pend_s0 = slot_cur;
+                        pend_s0 = slot;
                         pend_r0 = r;
                     } else if (pend_s1 < 0) {
-                        pend_s1 = slot_cur;
+                        pend_s1 = slot;
                         pend_r1 = r;
                     } else {
                         pend_all = true;
@@ -560,12 +559,43 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
                     }
                 }
             }
-            r_cur = r_next;
-            r_next = r_after;
-            slot_cur = slot_next;
+        };
+        if (kb < ke) {
+            const int k = kb + sub;
+            Inc r0 = inc[clampk(k)], r1 = inc[clampk(k + LPV)], r2 = inc[clampk(k + 2 * LPV)];
+            int id1[3];
+            {
+                int id0[3];
+                load_ids(r0, id0);
+                load_ids(r1, id1);
+                NbrData dA, dB;
+                int sA = -1, sB = -1;
+                load_data(r0, id0, dA, sA);
+                for (int k0 = kb;; k0 += 2 * LPV) {
+                    const int kc = k0 + sub;
+                    int id2[3], id3[3];
+                    // Round m (buffer A); stages for rounds m+1 (data), m+2 (ids), m+3 (record).
+                    load_data(r1, id1, dB, sB);
+                    load_ids(r2, id2);
+                    const Inc r3 = inc[clampk(kc + 3 * LPV)];
+                    round(r0, dA, sA, kc < ke);
+                    if (k0 + LPV >= ke)
+                        break;
+                    // Round m+1 (buffer B); stages for rounds m+2, m+3, m+4.
+                    load_data(r2, id2, dA, sA);
+                    load_ids(r3, id3);
+                    const Inc r4 = inc[clampk(kc + 4 * LPV)];
+                    round(r1, dB, sB, kc + LPV < ke);
+                    if (k0 + 2 * LPV >= ke)
+                        break;
+                    r0 = r2;
+                    r1 = r3;
+                    r2 = r4;
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
-                id_cur[j] = id_next[j];
+                    for (int j = 0; j < 3; ++j)
+                        id1[j] = id3[j];
+                }
+            }
         }
         if (KIND == 1) {
             // Second pass: indefinite tets add proj(H_e) as stored by the eigen-clamp kernel (for slot a: the 6
@@ -687,14 +717,13 @@ __device__ __forceinline__ int xcd_slot(int b, int G) {
     return x * q + (x < r ? x : r) + i;
 }
 
-// Waves per SIMD the register allocation must allow (168 VGPRs for 3): the common shapes fit without spilling;
-// one lane per vertex with a Projected Hessian and the HBM-accumulation tiles keep a 256-VGPR budget.
-constexpr int assemble_min_waves(int lpv, int kind, bool global_acc) {
-    return (global_acc || (lpv == 1 && kind == 1)) ? 2 : 3;
-}
+// Waves per SIMD the register allocation must allow: 2 (a 256-VGPR budget).  A tile's LDS segment (about 18 KB
+// for 16 vertices of degree 15) already holds a CU to 8 single-wave workgroups, i.e. 2 waves per SIMD, so a smaller
+// budget would only force the two-round pipeline's buffers to spill.
+constexpr int kAssembleMinWaves = 2;
 
 template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GLOBAL_ACC>
-__global__ __launch_bounds__(kWave* WAVES, assemble_min_waves(LPV, KIND, GLOBAL_ACC)) void k_assemble(AssembleParams p) {
+__global__ __launch_bounds__(kWave* WAVES, kAssembleMinWaves) void k_assemble(AssembleParams p) {
     extern __shared__ double lds[];
     const int slot = xcd_slot(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
     const int tile = p.tile_ids ? p.tile_ids[slot] : slot;
@@ -703,7 +732,7 @@ __global__ __launch_bounds__(kWave* WAVES, assemble_min_waves(LPV, KIND, GLOBAL_
 
 // Persistent form: a fixed grid walks the tile list with a grid stride (the batched launch).
 template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GLOBAL_ACC>
-__global__ __launch_bounds__(kWave* WAVES, 2) void k_assemble_persistent(AssembleParams p, int num_tiles) {
+__global__ __launch_bounds__(kWave* WAVES, kAssembleMinWaves) void k_assemble_persistent(AssembleParams p, int num_tiles) {
     extern __shared__ double lds[];
     for (int t = blockIdx.x; t < num_tiles; t += gridDim.x) {
         const int tile = p.tile_ids ? p.tile_ids[t] : t;
