@@ -90,6 +90,16 @@ lpg_status lpg_tet_energy_eval_device(lpg_tet_energy* energy, const double* x_de
 /* Number of elements whose projection changed them (indefinite material tangent) in the last Projected eval. */
 int32_t lpg_tet_energy_last_indefinite_count(const lpg_tet_energy* energy);
 
+/* Energy::max_feasible_step (energy/scalar/energy.hpp:34-36, energy.cpp:35-45): the largest alpha in [0, 1] such
+ * that every tet keeps det F > 0 on the whole segment x + t dx, t in [0, alpha] (a conservative bound: 0.9 of the
+ * bisected first unsafe t, positive_det_feasible_step.cpp:13-84), the minimum over tets
+ * (mesh_energy.cpp:88-107, volumetric_fem_energy.hpp:156-185, neo_hookean.cpp:130-145); 1 when nothing limits it.
+ * x must be feasible (det F > 0 everywhere: LPG_DOMAIN_ERROR otherwise); non-finite x or dx: LPG_INVALID_ARGUMENT.
+ * Host arrays of num_dofs doubles; the _device form takes device arrays and enqueues on `stream`. */
+lpg_status lpg_tet_energy_max_feasible_step(lpg_tet_energy* energy, const double* x, const double* dx, double* alpha);
+lpg_status lpg_tet_energy_max_feasible_step_device(lpg_tet_energy* energy, const double* x_dev, const double* dx_dev,
+                                                   double* alpha, void* stream);
+
 /* Differentiation form (device buffers).  Per-element parameter derivatives have layout num_tets x 2
  * (d/dlambda, d/dmu).  theta_dev may be NULL (snapshot theta).
  *   dE_dtheta      : dE/dtheta_e = w_e (L^2/2, (|F|^2-3)/2 - L), L = log det F (psi is linear in theta,

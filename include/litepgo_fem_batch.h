@@ -33,6 +33,11 @@ lpg_status lpg_tet_batch_eval_device(lpg_tet_energy* energy, const double* x_dev
                                      int32_t want_gradient, int32_t hessian_kind, double* energies_dev,
                                      double* gradient_dev, double* hessian_values_dev, void* stream);
 
+/* Per-mesh feasible steps: alphas_dev[num_meshes] (device) receives each mesh's max_feasible_step of x along dx
+ * (lpg_tet_energy_max_feasible_step restricted to the mesh's tets).  Synchronises on `stream` to read the status. */
+lpg_status lpg_tet_batch_max_feasible_step_device(lpg_tet_energy* energy, const double* x_dev, const double* dx_dev,
+                                                  double* alphas_dev, void* stream);
+
 /* Graph mode (any handle): the first device evaluation with a given set of buffers, kind and launch plan runs
  * eagerly; a repeat of it is captured into a hipGraph (kernels plus the status read-back) and later repeats
  * replay the graph, one launch per evaluation.  Changing buffers, kind or plan falls back to the eager path. */

@@ -24,6 +24,8 @@ SIGNATURES = {
     "lpg_tet_energy_eval": (ctypes.c_int, [_P, _P, _I32, _I32, _P, _P, _P]),
     "lpg_tet_energy_eval_device": (ctypes.c_int, [_P, _P, _P, _I32, _I32, _P, _P, _P, _P]),
     "lpg_tet_energy_last_indefinite_count": (_I32, [_P]),
+    "lpg_tet_energy_max_feasible_step": (ctypes.c_int, [_P, _P, _P, _P]),
+    "lpg_tet_energy_max_feasible_step_device": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "lpg_tet_energy_dE_dtheta_device": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "lpg_tet_energy_force_vjp_theta_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "lpg_tet_energy_force_jvp_theta_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
@@ -40,6 +42,7 @@ SIGNATURES = {
     "lpg_tet_batch_num_meshes": (_I32, [_P]),
     "lpg_tet_batch_offsets": (ctypes.c_int, [_P, _P, _P, _P]),
     "lpg_tet_batch_eval_device": (ctypes.c_int, [_P, _P, _P, _I32, _I32, _P, _P, _P, _P]),
+    "lpg_tet_batch_max_feasible_step_device": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "lpg_tet_energy_set_graph": (ctypes.c_int, [_P, _I32]),
     "lpg_tet_energy_autotune": (ctypes.c_int, [_P, _P, _I32, _I32, _P, _P]),
 }

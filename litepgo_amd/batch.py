@@ -88,3 +88,14 @@ class BatchedFemEnergy(VolumetricFemEnergy):
         _native.check(_native.lib().lpg_tet_batch_eval_device(self._h, _tptr(x), _tptr(th), int(bool(gradient)), kind,
                                                               _tptr(E), _tptr(g), _tptr(H), _stream()), "eval")
         return E, g, H
+
+    def max_feasible_step_per_mesh(self, x, dx, out=None):
+        """Each mesh's max_feasible_step of x along dx (device tensors), as a (num_meshes,) device tensor."""
+        import torch
+
+        self._check_tensor(x, self.num_dofs, "x")
+        self._check_tensor(dx, self.num_dofs, "dx")
+        a = out if out is not None else torch.empty(self.num_meshes, dtype=torch.float64, device=x.device)
+        _native.check(_native.lib().lpg_tet_batch_max_feasible_step_device(self._h, _tptr(x), _tptr(dx), _tptr(a),
+                                                                           _stream()), "max_feasible_step")
+        return a

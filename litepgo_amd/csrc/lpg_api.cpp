@@ -185,6 +185,11 @@ void ensure_device(lpg_tet_energy* h) {
         h->seg_tbeg = {0, s.num_tets};
     }
     build_energy_chunks(h->seg_vbeg, h->chunk_begin, h->seg_chunk);
+    upload(h->d_seg_tbeg, h->seg_tbeg.data(), h->seg_tbeg.size() * sizeof(int32_t));
+    {
+        const int32_t all[2] = {0, s.num_tets};
+        upload(h->d_seg_all, all, sizeof all);
+    }
     upload(h->d_chunk_begin, h->chunk_begin.data(), h->chunk_begin.size() * sizeof(int32_t));
     upload(h->d_seg_chunk, h->seg_chunk.data(), h->seg_chunk.size() * sizeof(int32_t));
     h->d_chunk_sum.ensure(std::max<size_t>(1, h->chunk_begin.size()) * sizeof(double));
@@ -212,6 +217,8 @@ void raise_status(unsigned status) {
         fail(kDomain, "MeshEnergy: non-finite projected Hessian");
     if (status & kStatusNonFiniteOut)
         fail(kDomain, "VolumetricElement3D: non-finite integrated outputs");
+    if (status & kStatusNonFiniteStep)
+        fail(kDomain, "VolumetricFemEnergy: non-finite input increment");
 }
 
 double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter) {
@@ -394,6 +401,28 @@ lpg_status derivative_call(lpg_tet_energy* h, const double* x, const double* the
 }
 
 } // namespace
+
+// Energy::max_feasible_step (energy.cpp:35-45) over MeshEnergy::feasible_step (mesh_energy.cpp:88-107): one bound per
+// segment (the whole handle, or each mesh of a batch), written to `alphas` on the device; returns the first.
+double feasible_step_call(lpg_tet_energy* h, const double* x, const double* dx, bool per_mesh, double* alphas,
+                          hipStream_t s) {
+    const auto& st = h->setup;
+    if ((!x || !dx) && st.num_dofs > 0)
+        fail(kInvalid, "Energy::max_feasible_step: x and dx must have num_dofs finite entries");
+    if (!st.theta_valid && st.num_tets > 0)
+        fail(kInvalid, "NeoHookean requires finite lambda >= 0 and mu > 0");
+    ensure_device(h);
+    const int segments = per_mesh ? static_cast<int>(h->seg_tbeg.size()) - 1 : 1;
+    double* out = alphas ? alphas : h->d_energy();
+    launch_feasible_step(h->mesh(), x, dx, per_mesh ? h->d_seg_tbeg.as<int32_t>() : h->d_seg_all.as<int32_t>(),
+                         segments, out, h->d_status(), s);
+    if (alphas) {
+        // Copy the first bound into the read-back slot so finish() returns it.
+        hip_check(hipMemcpyAsync(h->d_energy(), alphas, sizeof(double), hipMemcpyDeviceToDevice, s), "step copy");
+    }
+    return finish(h, s, false);
+}
+
 } // namespace lpg
 
 using lpg::fail;
@@ -545,6 +574,38 @@ lpg_status lpg_tet_energy_vjp_device(lpg_tet_energy* h, const double* x_dev, con
                                 [&](const lpg::DeviceMesh& m, unsigned* st, hipStream_t s) {
                                     lpg::launch_vjp(m, x_dev, theta_dev, gE_dev, w_dev, gx_dev, gtheta_dev, st, s);
                                 });
+}
+
+lpg_status lpg_tet_energy_max_feasible_step(lpg_tet_energy* h, const double* x, const double* dx, double* alpha) {
+    return guarded([&] {
+        if (!h || !alpha)
+            fail(lpg::kInvalid, "lpg_tet_energy_max_feasible_step: null argument");
+        lpg::BusyScope scope(h->busy);
+        const auto& s = h->setup;
+        if ((!x || !dx) && s.num_dofs > 0)
+            fail(lpg::kInvalid, "Energy::max_feasible_step: x and dx must have num_dofs finite entries");
+        for (int32_t i = 0; i < s.num_dofs; ++i)
+            if (!std::isfinite(x[i]))
+                fail(lpg::kInvalid, "Energy::max_feasible_step: x must have num_dofs finite entries");
+        for (int32_t i = 0; i < s.num_dofs; ++i)
+            if (!std::isfinite(dx[i]))
+                fail(lpg::kInvalid, "Energy::max_feasible_step: dx must have num_dofs finite entries");
+        lpg::ensure_device(h);
+        lpg::upload(h->d_x, x, s.num_dofs * sizeof(double));
+        lpg::upload(h->d_dx, dx, s.num_dofs * sizeof(double));
+        *alpha = lpg::feasible_step_call(h, h->d_x.as<double>(), h->d_dx.as<double>(), false, nullptr,
+                                         h->own_stream);
+    });
+}
+
+lpg_status lpg_tet_energy_max_feasible_step_device(lpg_tet_energy* h, const double* x_dev, const double* dx_dev,
+                                                   double* alpha, void* stream) {
+    return guarded([&] {
+        if (!h || !alpha)
+            fail(lpg::kInvalid, "lpg_tet_energy_max_feasible_step_device: null argument");
+        lpg::BusyScope scope(h->busy);
+        *alpha = lpg::feasible_step_call(h, x_dev, dx_dev, false, nullptr, static_cast<hipStream_t>(stream));
+    });
 }
 
 // ---- extensions (litepgo_fem_ext.h) ----

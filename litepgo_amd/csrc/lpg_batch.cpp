@@ -127,6 +127,16 @@ lpg_status lpg_tet_batch_eval_device(lpg_tet_energy* h, const double* x_dev, con
     });
 }
 
+lpg_status lpg_tet_batch_max_feasible_step_device(lpg_tet_energy* h, const double* x_dev, const double* dx_dev,
+                                                  double* alphas_dev, void* stream) {
+    return guarded([&] {
+        if (!h || !alphas_dev)
+            fail(lpg::kInvalid, "lpg_tet_batch_max_feasible_step_device: null argument");
+        lpg::BusyScope scope(h->busy);
+        (void)lpg::feasible_step_call(h, x_dev, dx_dev, true, alphas_dev, static_cast<hipStream_t>(stream));
+    });
+}
+
 lpg_status lpg_tet_energy_set_graph(lpg_tet_energy* h, int32_t enable) {
     return guarded([&] {
         if (!h)

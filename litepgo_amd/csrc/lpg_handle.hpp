@@ -92,6 +92,9 @@ struct lpg_tet_energy {
     lpg::DeviceBuffer d_vertex_E, d_chunk_begin, d_chunk_sum, d_seg_chunk;
     lpg::DeviceBuffer d_proj_slot, d_proj_list, d_proj_H;
     lpg::DeviceBuffer d_x, d_g, d_H; // staging for the host-pointer API
+    lpg::DeviceBuffer d_dx;          // staging of the step direction (host-pointer feasible step)
+    lpg::DeviceBuffer d_seg_tbeg;    // seg_tbeg on the device (per-mesh feasible steps)
+    lpg::DeviceBuffer d_seg_all;     // {0, T}: the whole handle as one segment
     lpg::TilePlan plan;
     int plan_version = 0;
     lpg::GraphCache graph;
@@ -136,6 +139,10 @@ double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter);
 double run_eval(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g, double* H,
                 double* energies, hipStream_t stream);
 // Accumulates the per-phase event times of the last eager evaluation (timing mode).
+// Feasible step of x along dx (device arrays): one bound per segment into `alphas` (device; null: the handle's
+// read-back slot), whole handle or per mesh; synchronises once and returns the first bound.
+double feasible_step_call(lpg_tet_energy* h, const double* x, const double* dx, bool per_mesh, double* alphas,
+                          hipStream_t s);
 void collect_timing(lpg_tet_energy* h);
 // The status mapping alone (status word as read back).
 void raise_status(unsigned status);

@@ -28,6 +28,7 @@ enum StatusBits : unsigned {
     kStatusDetF = 4u,           // det F <= 0 or non-finite kinematics           -> domain_error
     kStatusNonFiniteOut = 8u,   // non-finite element or assembled outputs       -> domain_error
     kStatusProjection = 16u,    // projection failed to converge / non-finite    -> domain_error
+    kStatusNonFiniteStep = 32u, // non-finite deformation-gradient increment     -> domain_error
 };
 
 // Device layout (no per-tet geometry is stored; Dm^-1 and w are recomputed from the rest positions):
@@ -175,6 +176,10 @@ void launch_hvp(const DeviceMesh& m, const double* x, const double* theta, const
 // output may be null.
 void launch_vjp(const DeviceMesh& m, const double* x, const double* theta, const double* gE, const double* w,
                 double* gx, double* gtheta, unsigned* status, hipStream_t stream);
+// Per-segment feasible steps (alphas: num_segments doubles on the device, min over the segment's tets, 1 when
+// none limits); also clears the status word.
+void launch_feasible_step(const DeviceMesh& m, const double* x, const double* dx, const int32_t* seg_tbeg,
+                          int num_segments, double* alphas, unsigned* status, hipStream_t stream);
 // theta_e = (lambda, mu)(E_e, nu_e) and its reverse product.
 void launch_lame(const double* E, const double* nu, int64_t n, double* theta, hipStream_t stream);
 void launch_lame_vjp(const double* E, const double* nu, const double* gtheta, int64_t n, double* gE, double* gnu,

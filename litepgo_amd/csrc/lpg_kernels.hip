@@ -217,18 +217,13 @@ template <> struct IncT<true> {
     }
 };
 
-// 1 / sqrt(x) and sqrt(x) for finite x > 0 from the hardware estimate refined by Newton steps (within an ulp or
-// so; only the eigen-clamp's rotation angles use them, where c^2 + s^2 = 1 to rounding is what matters).
+// 1 / sqrt(x) for finite x > 0 from the hardware estimate refined by Newton steps (within an ulp or so; only the
+// eigen-clamp's rotation angles use it, where c^2 + s^2 = 1 to rounding is what matters).
 __device__ __forceinline__ double rsqrt_nr(double x) {
     double y = __builtin_amdgcn_rsq(x);
     const double hx = 0.5 * x;
     y = y * fma(-hx * y, y, 1.5);
     return y * fma(-hx * y, y, 1.5);
-}
-__device__ __forceinline__ double sqrt_nr(double x) {
-    const double y = rsqrt_nr(x);
-    const double r = x * y;
-    return fma(0.5 * y, fma(-r, r, x), r);
 }
 
 // 1 / d from the hardware reciprocal estimate refined by two Newton steps (within an ulp of the quotient for the
@@ -1248,6 +1243,189 @@ __global__ __launch_bounds__(256) void k_lame_vjp(const double* __restrict__ E, 
     }
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// Feasible step: the largest alpha in [0, 1] (times the reference's 0.9 safety factor) such that det F(x + t dx) > 0
+// for every tet and every t in [0, alpha] (Energy::max_feasible_step, energy.cpp:35-45).
+// ---------------------------------------------------------------------------------------------------------------
+__device__ __forceinline__ double triple(const double a[3], const double b[3], const double c[3]) {
+    // a . (b x c)
+    return a[0] * (b[1] * c[2] - b[2] * c[1]) + a[1] * (b[2] * c[0] - b[0] * c[2]) + a[2] * (b[0] * c[1] - b[1] * c[0]);
+}
+
+// Largest safe step along A + t dA keeping det > 0 (positive_det_feasible_step.cpp:13-84): columns scaled by their
+// largest magnitude, det(A + t dA) = c0 + c1 t + c2 t^2 + c3 t^3, its stationary points split [0, 1] into monotone
+// pieces, the first piece whose right end is not safely positive is bisected to the last safe t, and 0.9 of it is
+// returned.  A and dA are row-major 3x3 (column j = entries j, 3 + j, 6 + j).  Requires det A > 0.
+__device__ double det_feasible_step(const double A[9], const double dA[9]) {
+    double a[3][3], d[3][3]; // a[j] = column j of A / scale_j
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        double scale = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            scale = fmax(scale, fmax(fabs(A[3 * i + j]), fabs(dA[3 * i + j])));
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            a[j][i] = A[3 * i + j] / scale;
+            d[j][i] = dA[3 * i + j] / scale;
+        }
+    }
+    const double c0 = triple(a[0], a[1], a[2]);
+    const double c1 = triple(d[0], a[1], a[2]) + triple(a[0], d[1], a[2]) + triple(a[0], a[1], d[2]);
+    const double c2 = triple(d[0], d[1], a[2]) + triple(d[0], a[1], d[2]) + triple(a[0], d[1], d[2]);
+    const double c3 = triple(d[0], d[1], d[2]);
+    if (!(c0 > 0.0))
+        return 0.0;
+    constexpr double kEps = 2.220446049250313e-16;
+    const auto value = [&](double t) { return fma(fma(fma(c3, t, c2), t, c1), t, c0); };
+    const auto tolerance = [&](double t) {
+        return 64.0 * kEps * (((fabs(c3) * t + fabs(c2)) * t + fabs(c1)) * t + c0);
+    };
+    // Knots: 0, 1 and the stationary points inside (0, 1), in increasing order.
+    double knot[4] = {0.0, 1.0, 2.0, 2.0};
+    int count = 2;
+    const auto add_knot = [&](double t) {
+        if (t > 0.0 && t < 1.0)
+            knot[count++] = t;
+    };
+    const double qa = 3.0 * c3, qb = 2.0 * c2, qc = c1;
+    if (qa == 0.0) {
+        if (qb != 0.0)
+            add_knot(-qc / qb);
+    } else {
+        double disc = fma(qb, qb, -4.0 * qa * qc);
+        const double err = 8.0 * kEps * (qb * qb + fabs(4.0 * qa * qc));
+        if (disc >= -err) {
+            disc = fmax(0.0, disc);
+            const double q = -0.5 * (qb + copysign(sqrt(disc), qb));
+            if (q == 0.0) {
+                add_knot(-qb / (2.0 * qa));
+            } else {
+                add_knot(q / qa);
+                add_knot(qc / q);
+            }
+        }
+    }
+    // Insertion sort of at most four values.
+    for (int i = 1; i < count; ++i)
+        for (int j = i; j > 0 && knot[j] < knot[j - 1]; --j) {
+            const double tmp = knot[j];
+            knot[j] = knot[j - 1];
+            knot[j - 1] = tmp;
+        }
+    for (int i = 1; i < count; ++i) {
+        double lo = knot[i - 1], hi = knot[i];
+        if (value(hi) > tolerance(hi))
+            continue;
+        // At most ~1100 halvings reach mid == lo or mid == hi for doubles in [0, 1].
+        for (;;) {
+            const double mid = lo + (hi - lo) / 2.0;
+            if (mid == lo || mid == hi)
+                break;
+            if (value(mid) > tolerance(mid))
+                lo = mid;
+            else
+                hi = mid;
+        }
+        return 0.9 * lo;
+    }
+    return 1.0;
+}
+
+struct StepParams {
+    const double* Xr;
+    const int32_t* conn;
+    const double* theta_snap;
+    const double* x;
+    const double* dx;
+    const int32_t* seg_tbeg; // num_segments + 1 tet offsets (one segment for a plain mesh)
+    int num_segments;
+    unsigned long long* alpha_bits; // per segment, bit patterns of non-negative doubles (ordered as integers)
+    unsigned* status;
+    int num_tets;
+    int num_dofs;
+};
+
+__global__ void k_step_init(unsigned long long* alpha_bits, int n, unsigned* status) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        alpha_bits[i] = static_cast<unsigned long long>(__double_as_longlong(1.0));
+    if (i == 0)
+        status[0] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_feasible_step(StepParams p) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    double alpha = 1.0;
+    int seg = 0;
+    unsigned flags = 0;
+    // Every entry of x and dx is checked (grid stride), also those of vertices that no tet uses.
+    for (int i = e; i < p.num_dofs; i += 256 * gridDim.x)
+        if (!(isfinite(p.x[i]) && isfinite(p.dx[i])))
+            flags |= kStatusNonFiniteX;
+    if (e < p.num_tets) {
+        int ids[4];
+        TetIn t;
+        load_tet(p.Xr, p.conn, p.theta_snap, p.x, nullptr, e, ids, t);
+        Geo geo;
+        rest_geometry(t, geo);
+        Kin k;
+        kinematics(t, geo.Di, k);
+        if (!kin_ok(k))
+            flags |= kStatusDetF;
+        if (!theta_ok(t.lam, t.mu))
+            flags |= kStatusBadTheta;
+        // dF = dDu Dm^-1 with dDu(i, c) = (dx_{c+1} - dx_0)_i.
+        double du[4][3];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                du[b][i] = p.dx[3 * static_cast<size_t>(ids[b]) + i];
+        double dF[9];
+        bool fin = true;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                double s = 0.0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    s = fma(du[c + 1][i] - du[0][i], geo.Di[3 * c + j], s);
+                dF[3 * i + j] = s;
+                fin = fin && isfinite(s);
+            }
+        if (!fin)
+            flags |= kStatusNonFiniteStep;
+        if (!flags)
+            alpha = det_feasible_step(k.F, dF);
+        int lo = 0, hi = p.num_segments; // segment of tet e: seg_tbeg[seg] <= e < seg_tbeg[seg + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (p.seg_tbeg[mid] <= e)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        seg = lo;
+    }
+    // Wave minimum when the whole wave lies in one segment (one atomic), else one atomic per lane.
+    const int seg0 = __shfl(seg, 0);
+    const bool uniform = __all(seg == seg0 || e >= p.num_tets);
+    if (uniform) {
+        double m = alpha;
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1)
+            m = fmin(m, __shfl_xor(m, off));
+        if ((threadIdx.x % kWave) == 0 && m < 1.0)
+            atomicMin(p.alpha_bits + seg0, static_cast<unsigned long long>(__double_as_longlong(m)));
+    } else if (e < p.num_tets && alpha < 1.0) {
+        atomicMin(p.alpha_bits + seg, static_cast<unsigned long long>(__double_as_longlong(alpha)));
+    }
+    if (flags)
+        atomicOr(p.status, flags);
+}
+
 void check_launch(const char* what) {
     const hipError_t err = hipGetLastError();
     if (err != hipSuccess)
@@ -1503,6 +1681,22 @@ void launch_lame_vjp(const double* E, const double* nu, const double* gtheta, in
     hipLaunchKernelGGL(k_lame_vjp, dim3(elementwise_blocks(n)), dim3(256), 0, stream, E, nu,
                        reinterpret_cast<const double2*>(gtheta), static_cast<long long>(n), gE, gnu);
     check_launch("k_lame_vjp");
+}
+
+void launch_feasible_step(const DeviceMesh& m, const double* x, const double* dx, const int32_t* seg_tbeg,
+                          int num_segments, double* alphas, unsigned* status, hipStream_t stream) {
+    auto* bits = reinterpret_cast<unsigned long long*>(alphas);
+    hipLaunchKernelGGL(k_step_init, dim3((num_segments + 255) / 256), dim3(256), 0, stream, bits, num_segments,
+                       status);
+    check_launch("k_step_init");
+    const int num_dofs = 3 * m.num_vertices;
+    if (m.num_tets == 0 && num_dofs == 0)
+        return;
+    StepParams p{m.Xr, m.conn, m.theta, x, dx, seg_tbeg, num_segments, bits, status, m.num_tets, num_dofs};
+    const int blocks = std::max(1, (std::max(m.num_tets, num_dofs) + 255) / 256);
+    hipLaunchKernelGGL(k_feasible_step, dim3(std::min(blocks, (m.num_tets + 255) / 256 + 1024)), dim3(256), 0,
+                       stream, p);
+    check_launch("k_feasible_step");
 }
 
 } // namespace lpg

@@ -231,6 +231,26 @@ class VolumetricFemEnergy:
     def eval_hessian(self, x, kind=HessianKind.Exact):
         return self.eval(x, hessian=kind)[2]
 
+    def max_feasible_step(self, x, dx):
+        """Largest alpha in [0, 1] keeping det F > 0 on x + t dx for t in [0, alpha] (Energy.max_feasible_step)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        dx = np.ascontiguousarray(dx, dtype=np.float64).reshape(-1)
+        if x.size != self.num_dofs or dx.size != self.num_dofs:
+            raise ValueError("Energy::max_feasible_step: x and dx must have num_dofs finite entries")
+        alpha = np.zeros(1)
+        _native.check(_native.lib().lpg_tet_energy_max_feasible_step(self._h, _ptr(x), _ptr(dx), _ptr(alpha)),
+                      "max_feasible_step")
+        return float(alpha[0])
+
+    def max_feasible_step_device(self, x, dx):
+        """max_feasible_step for float64 device tensors (one synchronisation)."""
+        self._check_tensor(x, self.num_dofs, "x")
+        self._check_tensor(dx, self.num_dofs, "dx")
+        alpha = np.zeros(1)
+        _native.check(_native.lib().lpg_tet_energy_max_feasible_step_device(self._h, _tptr(x), _tptr(dx), _ptr(alpha),
+                                                                            _stream()), "max_feasible_step")
+        return float(alpha[0])
+
     # ---- device interface (torch float64 tensors on the HIP device) --------------------------------------
     def _check_tensor(self, t, n, name):
         import torch

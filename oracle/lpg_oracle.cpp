@@ -521,9 +521,131 @@ void eval_element(const Energy& E, int e, const double* x, const double* theta_o
         fail(kDomain, "VolumetricElement3D: non-finite integrated outputs");
 }
 
+// eval_positive_det_feasible_step (positive_det_feasible_step.cpp:13-84) on column-major 3x3 A, dA: scale each
+// column pair by its largest magnitude, expand det(a + t d) = c0 + c1 t + c2 t^2 + c3 t^3 (c1, c2 by mixed triple
+// products), split [0, 1] at the stationary points in (0, 1), bisect the first piece whose right end is not above
+// 64 eps of the magnitude polynomial, and return 0.9 of the last safe t (1 if none).
+double cross_dot(const double* a, const double* b, const double* c) {
+    // a . (b x c)
+    const double x0 = b[1] * c[2] - b[2] * c[1];
+    const double x1 = b[2] * c[0] - b[0] * c[2];
+    const double x2 = b[0] * c[1] - b[1] * c[0];
+    return a[0] * x0 + a[1] * x1 + a[2] * x2;
+}
+
+double positive_det_step(const double* A, const double* dA) {
+    if (!finite(A, 9) || !finite(dA, 9))
+        fail(kInvalid, "Positive determinant step requires finite matrices");
+    const double det_A = det3(A);
+    if (!std::isfinite(det_A) || det_A <= 0)
+        fail(kDomain, "Positive determinant step requires finite det(A)>0");
+    double a[9], d[9];
+    for (int j = 0; j < 3; ++j) {
+        double scale = 0;
+        for (int i = 0; i < 3; ++i)
+            scale = std::max(scale, std::max(std::abs(A[i + 3 * j]), std::abs(dA[i + 3 * j])));
+        for (int i = 0; i < 3; ++i) {
+            a[i + 3 * j] = A[i + 3 * j] / scale;
+            d[i + 3 * j] = dA[i + 3 * j] / scale;
+        }
+    }
+    const double *a0 = a, *a1 = a + 3, *a2 = a + 6, *d0 = d, *d1 = d + 3, *d2 = d + 6;
+    const double c0 = det3(a);
+    const double c1 = cross_dot(d0, a1, a2) + cross_dot(a0, d1, a2) + cross_dot(a0, a1, d2);
+    const double c2 = cross_dot(d0, d1, a2) + cross_dot(d0, a1, d2) + cross_dot(a0, d1, d2);
+    const double c3 = det3(d);
+    if (!(c0 > 0))
+        return 0;
+    const double eps = std::numeric_limits<double>::epsilon();
+    const auto value = [&](double t) { return std::fma(std::fma(std::fma(c3, t, c2), t, c1), t, c0); };
+    const auto tolerance = [&](double t) {
+        return 64 * eps * (((std::abs(c3) * t + std::abs(c2)) * t + std::abs(c1)) * t + c0);
+    };
+    std::vector<double> knots{0.0, 1.0};
+    const auto add_knot = [&](double t) {
+        if (t > 0 && t < 1)
+            knots.push_back(t);
+    };
+    // Stationary points: roots of 3 c3 t^2 + 2 c2 t + c1, by the cancellation-free quadratic formula.
+    const double qa = 3 * c3, qb = 2 * c2, qc = c1;
+    if (qa == 0) {
+        if (qb != 0)
+            add_knot(-qc / qb);
+    } else {
+        double disc = std::fma(qb, qb, -4 * qa * qc);
+        const double error = 8 * eps * (qb * qb + std::abs(4 * qa * qc));
+        if (disc >= -error) {
+            disc = std::max(0.0, disc);
+            const double q = -0.5 * (qb + std::copysign(std::sqrt(disc), qb));
+            if (q == 0) {
+                add_knot(-qb / (2 * qa));
+            } else {
+                add_knot(q / qa);
+                add_knot(qc / q);
+            }
+        }
+    }
+    std::sort(knots.begin(), knots.end());
+    for (size_t i = 1; i < knots.size(); ++i) {
+        double lo = knots[i - 1], hi = knots[i];
+        if (value(hi) > tolerance(hi))
+            continue;
+        for (;;) {
+            const double mid = lo + (hi - lo) / 2;
+            if (mid == lo || mid == hi)
+                break;
+            if (value(mid) > tolerance(mid))
+                lo = mid;
+            else
+                hi = mid;
+        }
+        return 0.9 * lo;
+    }
+    return 1;
+}
+
 void check_x(const Energy& E, const double* x) {
     if (!finite(x, E.num_dofs))
         fail(kInvalid, "Energy::eval: x must have num_dofs finite entries");
+}
+
+double feasible_step(Energy& E, const double* x, const double* dx) {
+    if (!finite(x, E.num_dofs))
+        fail(kInvalid, "Energy::max_feasible_step: x must have num_dofs finite entries");
+    if (!finite(dx, E.num_dofs))
+        fail(kInvalid, "Energy::max_feasible_step: dx must have num_dofs finite entries");
+    double bound = 1.0;
+    for (int e = 0; e < E.num_tets; ++e) {
+        double c[12], dc[12];
+        for (int l = 0; l < 12; ++l) {
+            c[l] = x[dof(E, e, l)];
+            dc[l] = dx[dof(E, e, l)];
+        }
+        const double* Be = &E.B[108 * static_cast<int64_t>(e)];
+        double F[9], dF[9];
+        for (int r = 0; r < 9; ++r) {
+            double s = 0, ds = 0;
+            for (int l = 0; l < 12; ++l) {
+                s += Be[r * 12 + l] * c[l];
+                ds += Be[r * 12 + l] * dc[l];
+            }
+            F[r] = (r % 4 == 0 ? 1.0 : 0.0) + s;
+            dF[r] = ds;
+        }
+        if (!finite(F, 9))
+            fail(kDomain, "VolumetricFemEnergy: non-finite deformation gradient");
+        if (!finite(dF, 9))
+            fail(kDomain, "VolumetricFemEnergy: non-finite input increment");
+        const double* Fp = &E.Fp[9 * static_cast<int64_t>(e)];
+        MaterialOut m;
+        neo_hookean(F, Fp, &E.theta[2 * e], false, false, false, m); // eval_W: validates the point
+        const double zero[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const double step = std::min(positive_det_step(F, dF), positive_det_step(Fp, zero));
+        if (!(step >= 0 && step <= 1))
+            fail(kLogic, "VolumetricElement3D: local feasible step outside [0,1]");
+        bound = std::min(bound, step);
+    }
+    return bound;
 }
 
 struct BusyScope {
@@ -696,6 +818,19 @@ void orc_weights(void* h, double* w) {
 int orc_eval(void* h, const double* x, const double* theta_override, int want_g, int kind, int nthreads,
              double* value, double* g, double* Hval, char* err, int errlen) {
     ORACLE_TRY({ *value = eval(*static_cast<Energy*>(h), x, theta_override, want_g != 0, kind, nthreads, g, Hval); })
+}
+
+// eval_positive_det_feasible_step on one matrix pair (column-major), for the reference's own known answers.
+int orc_det_step(const double* A, const double* dA, double* alpha, char* err, int errlen) {
+    ORACLE_TRY({ *alpha = positive_det_step(A, dA); })
+}
+
+// Energy::max_feasible_step (energy.cpp:35-45) -> MeshEnergy::feasible_step (mesh_energy.cpp:88-107): the minimum,
+// starting from 1, over elements of VolumetricFemEnergy's per-element bound (volumetric_fem_energy.hpp:171-181):
+// F = I + reshape(B c), dF = reshape(B dc), NeoHookean::eval_feasible_step (neo_hookean.cpp:130-145) with a zero
+// Fp and theta increment, i.e. min(step(F, dF), step(Fp, 0)) after eval_W's validation of the point.
+int orc_feasible_step(void* h, const double* x, const double* dx, double* alpha, char* err, int errlen) {
+    ORACLE_TRY({ *alpha = feasible_step(*static_cast<Energy*>(h), x, dx); })
 }
 
 // Per-element energies E_e = w W(F_e), element order (the values_ of mesh_energy.cpp:71).

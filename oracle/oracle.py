@@ -61,6 +61,8 @@ def lib():
         L.orc_element_energies.argtypes = [P, P, P, ctypes.c_char_p, ctypes.c_int]
         L.orc_dE_dtheta.argtypes = [P, P, P, ctypes.c_char_p, ctypes.c_int]
         L.orc_force_theta_products.argtypes = [P, P, P, P, P, P, ctypes.c_char_p, ctypes.c_int]
+        L.orc_det_step.argtypes = [P, P, P, ctypes.c_char_p, ctypes.c_int]
+        L.orc_feasible_step.argtypes = [P, P, P, P, ctypes.c_char_p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -88,6 +90,16 @@ def neohookean(F, Fp, theta, *, P=True, A=True, mixed=True):
     return (W[0], None if Pm is None else Pm.reshape(3, 3, order="F"),
             None if Am is None else Am.reshape(9, 9, order="F"),
             None if Mx is None else Mx.reshape(9, 2, order="F"))
+
+
+def det_step(A, dA):
+    """eval_positive_det_feasible_step(A, dA) for 3x3 arrays (positive_det_feasible_step.cpp:13-84)."""
+    A = np.asarray(A, dtype=np.float64).reshape(-1, order="F").copy()
+    dA = np.asarray(dA, dtype=np.float64).reshape(-1, order="F").copy()
+    alpha = np.zeros(1)
+    buf = ctypes.create_string_buffer(512)
+    _check(lib().orc_det_step(_ptr(A), _ptr(dA), _ptr(alpha), buf, 512), buf)
+    return float(alpha[0])
 
 
 def project_psd(H):
@@ -179,6 +191,14 @@ class OracleEnergy:
         buf = ctypes.create_string_buffer(512)
         _check(lib().orc_dE_dtheta(self._h, _ptr(x), _ptr(out), buf, 512), buf)
         return out
+
+    def max_feasible_step(self, x, dx):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        dx = np.ascontiguousarray(dx, dtype=np.float64)
+        alpha = np.zeros(1)
+        buf = ctypes.create_string_buffer(512)
+        _check(lib().orc_feasible_step(self._h, _ptr(x), _ptr(dx), _ptr(alpha), buf, 512), buf)
+        return float(alpha[0])
 
     def force_theta_vjp(self, x, w):
         x = np.ascontiguousarray(x, dtype=np.float64)
