@@ -678,7 +678,22 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
     }
     if (KIND >= 0) {
         if (!GLOBAL_ACC) {
-            for (int i = threadIdx.x; i < seg1 - seg0; i += kThreads) {
+            // Four LDS reads in flight per lane before their stores: one LDS latency per four values.
+            const int n = seg1 - seg0;
+            int i = threadIdx.x;
+            for (; i + 3 * kThreads < n; i += 4 * kThreads) {
+                double val[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    val[q] = acc[i + q * kThreads];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (!isfinite(val[q]))
+                        flags |= kStatusNonFiniteOut;
+                    p.H[seg0 + i + q * kThreads] = val[q];
+                }
+            }
+            for (; i < n; i += kThreads) {
                 const double val = acc[i];
                 if (!isfinite(val))
                     flags |= kStatusNonFiniteOut;
