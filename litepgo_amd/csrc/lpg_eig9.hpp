@@ -191,5 +191,301 @@ LPG_EIG_FN bool sym_eig(double a[45], double d[N], double V[N][N], double scale,
     return ok;
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// PSD part without the full eigenbasis.
+//
+// proj(M) = V max(Lambda, 0) V^T = M + sum_{lambda_q < 0} |lambda_q| v_q v_q^T = sum_{lambda_q > 0} lambda_q v_q v_q^T.
+// The eigenvalues come from the QL iteration on the tridiagonal form alone (no basis rotations), and eigenvectors
+// are computed only for the smaller of the negative and positive sets (at most 4 of 9): inverse iteration on the
+// tridiagonal matrix T - lambda_q I (Gaussian elimination with partial pivoting, three steps), orthogonalised
+// against the vectors already found (modified Gram-Schmidt, so clustered eigenvalues give an orthonormal basis of
+// their eigenspace), then mapped back through the Householder reflectors.  Errors: a vector's angle error
+// ~ eps |T| / gap only mixes it with its neighbours; within the selected set the mixing cancels in the sum up to
+// the cluster width, and a neighbour across zero has |lambda| <= gap, so proj(M) is accurate to a few eps |M|.
+// ---------------------------------------------------------------------------------------------------------------
+
+// Start vectors of the inverse iterations (fixed, generic values in [0.5, 1.5], one row per eigenvalue index).
+constexpr double kStartVec[N][N] = {
+    {1.06, 0.71, 1.33, 0.58, 1.21, 0.93, 1.47, 0.66, 1.12}, {0.84, 1.29, 0.62, 1.41, 0.77, 1.18, 0.53, 1.36, 0.98},
+    {1.24, 0.57, 1.09, 0.88, 1.45, 0.69, 1.02, 1.31, 0.61}, {0.73, 1.38, 0.95, 0.52, 1.16, 1.27, 0.81, 0.64, 1.43},
+    {1.35, 0.92, 0.68, 1.22, 0.59, 1.04, 1.39, 0.86, 0.74}, {0.55, 1.11, 1.42, 0.97, 1.30, 0.63, 0.76, 1.19, 1.05},
+    {1.18, 0.83, 0.56, 1.34, 1.01, 1.46, 0.67, 0.91, 1.26}, {0.96, 1.49, 1.14, 0.72, 0.85, 0.54, 1.23, 1.07, 0.79},
+    {1.44, 0.65, 0.89, 1.08, 0.74, 1.37, 1.15, 0.51, 1.28}};
+
+// Number of values psd_part keeps in its spill store between the reduction and the back-transformation: the
+// Householder vectors (35) and their coefficients (7).
+constexpr int kSpill = (N - 1) * N / 2 - 1 + (N - 2);
+
+// a: packed upper triangle of the symmetric matrix M (destroyed).  On return proj(M) = (add_M ? M : 0) + C, with C
+// packed like a.  `spill` provides put(i, value) / get(i) for kSpill doubles (LDS on the device), which hold the
+// reflectors while the inverse iterations run.  Returns false if a QL sweep did not converge.
+template <class Rsqrt, class Recip, class Spill>
+LPG_EIG_FN bool psd_part(double a[45], double C[45], bool& add_M, double scale, Rsqrt rsqrt, Recip recip,
+                         Spill& spill) {
+    double e[N], d[N];
+    // ---- Householder reduction; reflector k (vector on indices k+1..N-1, coefficient beta_k) goes to the spill
+    // store at offset off(k) = sum_{k' < k} (N - 1 - k'), the coefficients after the vectors.
+#pragma unroll
+    for (int k = 0; k < N - 2; ++k) {
+        const int m = N - 1 - k;
+        const int off = k * (N - 1) - k * (k - 1) / 2;
+        double v[N];
+        double sigma = 0.0;
+#pragma unroll
+        for (int i = 1; i < N; ++i)
+            if (i < m) {
+                v[i] = a[sym(k, k + 1 + i)];
+                sigma = fma(v[i], v[i], sigma);
+            }
+        const double x0 = a[sym(k, k + 1)];
+        const double nx = sigma > 0.0 ? fma(x0, x0, sigma) * rsqrt(fma(x0, x0, sigma)) : fabs(x0);
+        const double alpha = x0 > 0.0 ? -nx : nx;
+        v[0] = x0 - alpha;
+        const double vtv = fma(v[0], v[0], sigma);
+        const double bk = sigma > 0.0 ? 2.0 * recip(vtv) : 0.0;
+        e[k] = sigma > 0.0 ? alpha : x0;
+        double pw[N];
+        double pv = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if (i < m) {
+                double s = 0.0;
+#pragma unroll
+                for (int j = 0; j < N; ++j)
+                    if (j < m)
+                        s = fma(a[sym(k + 1 + i, k + 1 + j)], v[j], s);
+                pw[i] = bk * s;
+                pv = fma(pw[i], v[i], pv);
+            }
+        const double Kc = 0.5 * bk * pv;
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if (i < m)
+                pw[i] = fma(-Kc, v[i], pw[i]);
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+                if (i < m && j < m && j >= i)
+                    a[sym(k + 1 + i, k + 1 + j)] =
+                        fma(-v[i], pw[j], fma(-pw[i], v[j], a[sym(k + 1 + i, k + 1 + j)]));
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if (i < m)
+                spill.put(off + i, v[i]);
+        spill.put(kSpill - (N - 2) + k, bk);
+    }
+    e[N - 2] = a[sym(N - 2, N - 1)];
+    e[N - 1] = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        d[i] = a[sym(i, i)];
+    double d0[N], e0[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        d0[i] = d[i];
+        e0[i] = e[i];
+    }
+
+    // ---- Eigenvalues: implicit QL on (d, e), as in sym_eig but without the basis.
+    const double tiny = 4e-16 * scale;
+    bool ok = true;
+#pragma unroll
+    for (int l = 0; l < N - 1; ++l) {
+        for (int it = 0; it < 40; ++it) {
+            int m = N - 1;
+#pragma unroll
+            for (int i = N - 2; i >= l; --i)
+                if (fabs(e[i]) <= tiny)
+                    m = i;
+            const bool active = m != l;
+            if (!any_lane(active))
+                break;
+            if (it == 39)
+                ok = ok && !active;
+            double g = (d[l + 1] - d[l]) * (0.5 * recip(e[l]));
+            double r = fma(g, g, 1.0) * rsqrt(fma(g, g, 1.0));
+            double dm = d[N - 1];
+#pragma unroll
+            for (int i = N - 2; i >= l; --i)
+                if (i == m)
+                    dm = d[i];
+            g = dm - d[l] + e[l] * recip(g + (g >= 0.0 ? r : -r));
+            double s = 1.0, c = 1.0, p = 0.0;
+            bool alive = active;
+#pragma unroll
+            for (int i = N - 2; i >= l; --i) {
+                if (alive && i < m) {
+                    const double f = s * e[i];
+                    const double b = c * e[i];
+                    const double rr2 = fma(f, f, g * g);
+                    if (rr2 == 0.0) {
+                        d[i + 1] -= p;
+                        alive = false;
+                    } else {
+                        const double ir = rsqrt(rr2);
+                        e[i + 1] = rr2 * ir;
+                        s = f * ir;
+                        c = g * ir;
+                        g = d[i + 1] - p;
+                        const double rq = fma(d[i] - g, s, 2.0 * c * b);
+                        p = s * rq;
+                        d[i + 1] = g + p;
+                        g = fma(c, rq, -b);
+                    }
+                }
+            }
+            if (alive) {
+                d[l] -= p;
+                e[l] = g;
+            }
+#pragma unroll
+            for (int i = N - 1; i >= l; --i)
+                if (active && i == m)
+                    e[i] = 0.0;
+        }
+    }
+
+    // ---- The smaller eigen-set: at most 4 eigenvalues (negative ones when there are at most 4, else the
+    // positive ones, of which there are then at most 4), compacted in index order.
+    int nneg = 0;
+#pragma unroll
+    for (int q = 0; q < N; ++q)
+        nneg += d[q] < 0.0 ? 1 : 0;
+    const bool use_neg = nneg <= 4;
+    add_M = use_neg;
+    double sl[4] = {0.0, 0.0, 0.0, 0.0};
+    int ns = 0;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const bool sel = use_neg ? d[q] < 0.0 : d[q] > 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (sel && j == ns)
+                sl[j] = d[q];
+        ns += sel ? 1 : 0;
+    }
+
+    // ---- Their vectors in the tridiagonal basis by inverse iteration, orthogonalised against the earlier ones.
+    const double piv_floor = 2.220446049250313e-16 * scale;
+    double Y[4][N];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            Y[j][i] = 0.0;
+        if (!any_lane(j < ns))
+            continue;
+        const double lq = sl[j];
+        // T - lq I = P L U by Gaussian elimination with partial pivoting (U has two superdiagonals): per step the
+        // pivot row, the multiplier and whether rows were swapped.
+        double u1[N], u2[N], iu[N], mult[N - 1];
+        bool swp[N - 1];
+        double c0 = d0[0] - lq, c1 = e0[0];
+#pragma unroll
+        for (int i = 0; i < N - 1; ++i) {
+            const double n0 = e0[i], n1 = d0[i + 1] - lq, n2 = i + 2 < N ? e0[i + 1] : 0.0;
+            const bool sw = fabs(n0) > fabs(c0);
+            double p0 = sw ? n0 : c0;
+            const double p1 = sw ? n1 : c1, p2 = sw ? n2 : 0.0;
+            const double o0 = sw ? c0 : n0, o1 = sw ? c1 : n1, o2 = sw ? 0.0 : n2;
+            if (fabs(p0) < piv_floor)
+                p0 = p0 < 0.0 ? -piv_floor : piv_floor;
+            iu[i] = recip(p0);
+            const double mu = o0 * iu[i];
+            u1[i] = p1;
+            u2[i] = p2;
+            mult[i] = mu;
+            swp[i] = sw;
+            c0 = fma(-mu, p1, o1);
+            c1 = fma(-mu, p2, o2);
+        }
+        if (fabs(c0) < piv_floor)
+            c0 = c0 < 0.0 ? -piv_floor : piv_floor;
+        iu[N - 1] = recip(c0);
+        double y[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            y[i] = kStartVec[j][i];
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            // Modified Gram-Schmidt against the vectors already found, then normalisation.
+#pragma unroll
+            for (int jj = 0; jj < j; ++jj) {
+                double dot = 0.0;
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+                    dot = fma(y[i], Y[jj][i], dot);
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+                    y[i] = fma(-dot, Y[jj][i], y[i]);
+            }
+            double nrm = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                nrm = fma(y[i], y[i], nrm);
+            const double inv = rsqrt(nrm);
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                y[i] *= inv;
+            if (it == 3)
+                break;
+            // y <- (T - lq I)^-1 y: forward elimination with the recorded swaps, then back substitution.
+#pragma unroll
+            for (int i = 0; i < N - 1; ++i) {
+                const double zi = y[i], zj = y[i + 1];
+                y[i] = swp[i] ? zj : zi;
+                y[i + 1] = fma(-mult[i], y[i], swp[i] ? zi : zj);
+            }
+            y[N - 1] = y[N - 1] * iu[N - 1];
+            y[N - 2] = fma(-u1[N - 2], y[N - 1], y[N - 2]) * iu[N - 2];
+#pragma unroll
+            for (int i = N - 3; i >= 0; --i)
+                y[i] = fma(-u2[i], y[i + 2], fma(-u1[i], y[i + 1], y[i])) * iu[i];
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            Y[j][i] = y[i];
+    }
+
+    // ---- Back to the basis of M (x = H_0 H_1 ... H_{N-3} y) and C = sum_j |lambda_j| x_j x_j^T.
+#pragma unroll
+    for (int i = 0; i < 45; ++i)
+        C[i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (!any_lane(j < ns))
+            break;
+        double x[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            x[i] = j < ns ? Y[j][i] : 0.0; // lanes with fewer vectors add nothing (and no NaN)
+#pragma unroll
+        for (int k = N - 3; k >= 0; --k) {
+            const int m = N - 1 - k;
+            const int off = k * (N - 1) - k * (k - 1) / 2;
+            double t = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                if (i < m)
+                    t = fma(spill.get(off + i), x[k + 1 + i], t);
+            t *= spill.get(kSpill - (N - 2) + k);
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                if (i < m)
+                    x[k + 1 + i] = fma(-t, spill.get(off + i), x[k + 1 + i]);
+        }
+        const double w = j < ns ? fabs(sl[j]) : 0.0;
+#pragma unroll
+        for (int r = 0; r < N; ++r) {
+            const double wx = w * x[r];
+#pragma unroll
+            for (int c = r; c < N; ++c)
+                C[sym(r, c)] = fma(wx, x[c], C[sym(r, c)]);
+        }
+    }
+    return ok;
+}
+
 } // namespace eig9
 } // namespace lpg

@@ -906,70 +906,80 @@ __global__ __launch_bounds__(kPrepassThreads) void k_project_prepass(ProjectPara
         p.proj_slot[e] = slot;
 }
 
+// The reduced element Hessian M = w (C (x) I3) A (C (x) I3)^T of tet e, packed upper: with c_m = sum_a P(a,m) g_a
+// and r_m = F^-T c_m, M[(3m+i), (3n+k)] = w [mu (c_m . c_n) d_ik + lambda r_m,i r_n,k - p r_n,i r_m,k].
+__device__ __forceinline__ void reduced_hessian(const ProjectParams& p, int e, double (&M)[45]) {
+    int ids[4];
+    TetIn t;
+    load_tet(p.Xr, p.conn, p.theta_snap, p.x, p.theta, e, ids, t);
+    Geo geo;
+    rest_geometry(t, geo);
+    Kin k;
+    kinematics(t, geo.Di, k);
+    double G[4][3];
+    shape_gradients(geo.Di, G);
+    const double pr = fma(t.lam, k.L, -t.mu);
+    double cm[3][3], rm[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            cm[m][j] = fma(kP[3][m], G[3][j], fma(kP[2][m], G[2][j], fma(kP[1][m], G[1][j], kP[0][m] * G[0][j])));
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            rm[m][i] = dot3(k.a[3 * i], k.a[3 * i + 1], k.a[3 * i + 2], cm[m][0], cm[m][1], cm[m][2]);
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+#pragma unroll
+        for (int c = r; c < 9; ++c) {
+            const int m = r / 3, i = r % 3, n = c / 3, kk = c % 3;
+            double val = fma(t.lam * rm[m][i], rm[n][kk], -(pr * rm[n][i] * rm[m][kk]));
+            if (i == kk)
+                val = fma(t.mu, dot3(cm[m][0], cm[m][1], cm[m][2], cm[n][0], cm[n][1], cm[n][2]), val);
+            M[sym_index(r, c)] = geo.w * val;
+        }
+}
+
+// Per-lane spill store of the eigen-clamp (the Householder reflectors, eig9::kSpill doubles), lane-interleaved in
+// LDS so that the 64 lanes of a wave touch 64 consecutive doubles per access.
+struct LdsSpill {
+    double* base; // &lds[lane]
+    __device__ void put(int i, double v) { base[i * 64] = v; }
+    __device__ double get(int i) const { return base[i * 64]; }
+};
+
 // One thread per indefinite element (grid-stride over the compacted list).
-__global__ __launch_bounds__(64) void k_project_jacobi(ProjectParams p) {
+__global__ __launch_bounds__(64, 2) void k_project_jacobi(ProjectParams p) {
+    __shared__ double spill_lds[eig9::kSpill * 64];
     const unsigned count = *p.counter;
     for (unsigned idx = blockIdx.x * 64 + threadIdx.x; idx < count; idx += gridDim.x * 64) {
         const int e = p.proj_list[idx];
-        int ids[4];
-        TetIn t;
-        load_tet(p.Xr, p.conn, p.theta_snap, p.x, p.theta, e, ids, t);
-        Geo geo;
-        rest_geometry(t, geo);
-        Kin k;
-        kinematics(t, geo.Di, k);
-        double G[4][3];
-        shape_gradients(geo.Di, G);
-        const double pr = fma(t.lam, k.L, -t.mu);
-        // c_m = sum_a P(a,m) g_a, r_m = F^-T c_m.
-        double cm[3][3], rm[3][3];
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-                cm[m][j] = fma(kP[3][m], G[3][j],
-                               fma(kP[2][m], G[2][j], fma(kP[1][m], G[1][j], kP[0][m] * G[0][j])));
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-                rm[m][i] = dot3(k.a[3 * i], k.a[3 * i + 1], k.a[3 * i + 2], cm[m][0], cm[m][1], cm[m][2]);
-        // M[(3m+i), (3n+k)] = w [mu (c_m . c_n) d_ik + lambda r_m,i r_n,k - p r_n,i r_m,k], packed upper.
+        // The reduced Hessian is built twice (before the eigen-analysis and, when the projection adds to M, after
+        // it) rather than kept live through it, which keeps the eigen phase within the register budget.
+        const auto build_M = [&](double (&M)[45]) { reduced_hessian(p, e, M); };
         double M[45];
-#pragma unroll
-        for (int r = 0; r < 9; ++r)
-#pragma unroll
-            for (int c = r; c < 9; ++c) {
-                const int m = r / 3, i = r % 3, n = c / 3, kk = c % 3;
-                double val = fma(t.lam * rm[m][i], rm[n][kk], -(pr * rm[n][i] * rm[m][kk]));
-                if (i == kk)
-                    val = fma(t.mu, dot3(cm[m][0], cm[m][1], cm[m][2], cm[n][0], cm[n][1], cm[n][2]), val);
-                M[sym_index(r, c)] = geo.w * val;
-            }
+        build_M(M);
         double scale = 0.0;
 #pragma unroll
         for (int q = 0; q < 45; ++q)
             scale = fmax(scale, fabs(M[q]));
-        // Eigen-decomposition by Householder tridiagonalization and implicit QL (in place: M is destroyed).
-        double lam[9], V[9][9];
-        const bool converged = eig9::sym_eig(
-            M, lam, V, scale, [](double x) { return rsqrt_nr(x); }, [](double x) { return recip(x); });
-#pragma unroll
-        for (int r = 0; r < 9; ++r)
-            lam[r] = fmax(lam[r], 0.0);
-        // proj(M) = V max(Lambda, 0) V^T, packed upper (in place of M).
+        // proj(M) = (add_M ? M : 0) + C from the eigenvalues and the smaller eigen-set's vectors (M is destroyed
+        // by the reduction and rebuilt when needed).
+        double Cp[45];
+        bool add_M = false;
+        LdsSpill spill{spill_lds + threadIdx.x};
+        const bool converged = eig9::psd_part(
+            M, Cp, add_M, scale, [](double x) { return rsqrt_nr(x); }, [](double x) { return recip(x); }, spill);
+        if (add_M)
+            build_M(M);
         bool fin = true;
 #pragma unroll
-        for (int r = 0; r < 9; ++r)
-#pragma unroll
-            for (int c = r; c < 9; ++c) {
-                double s = 0.0;
-#pragma unroll
-                for (int q = 0; q < 9; ++q)
-                    s = fma(V[r][q] * lam[q], V[c][q], s);
-                M[sym_index(r, c)] = s;
-                fin = fin && isfinite(s);
-            }
+        for (int q = 0; q < 45; ++q) {
+            M[q] = add_M ? M[q] + Cp[q] : Cp[q];
+            fin = fin && isfinite(M[q]);
+        }
         // proj(H)_ab = sum_mn P(a,m) P(b,n) proj(M)[(m,i),(n,k)].  For each slot a, store the 6 distinct entries of
         // the symmetric block (a, a) and the blocks (a, a^j), j = 1..3 -- the values the incidence of the tet's
         // vertex a adds to its rows.
@@ -1600,7 +1610,9 @@ void launch_project_jacobi(const DeviceMesh& m, const EvalArgs& a, hipStream_t s
     if (m.num_tets == 0)
         return;
     ProjectParams p{m.Xr, m.conn, m.theta, a.x, a.theta, m.num_tets, a.status, a.counter, a.proj_slot, a.proj_list, a.proj_H};
-    const int blocks = std::min(4096, (m.num_tets + 63) / 64);
+    // Enough single-wave workgroups for two waves per SIMD on 256 CUs; the list length is only known on the device,
+    // so surplus workgroups exit at once and fewer elements than workgroups leave the rest idle.
+    const int blocks = std::min(8192, (m.num_tets + 63) / 64);
     hipLaunchKernelGGL(k_project_jacobi, dim3(blocks), dim3(64), 0, stream, p);
     check_launch("k_project_jacobi");
 }

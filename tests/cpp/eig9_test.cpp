@@ -1,6 +1,7 @@
-// CPU test of the 9x9 symmetric eigen-solver used by the PSD projection kernel (litepgo_amd/csrc/lpg_eig9.hpp):
-// orthonormal eigenvectors, reconstruction, and eigenvalues against a cyclic Jacobi reference, on random symmetric
-// matrices, matrices with repeated and zero eigenvalues, and reduced neo-Hookean element Hessians.
+// CPU test of the 9x9 symmetric eigen-solvers used by the PSD projection kernel (litepgo_amd/csrc/lpg_eig9.hpp):
+// orthonormal eigenvectors, reconstruction, and eigenvalues against a cyclic Jacobi reference, and the lean PSD
+// part (psd_part) against the full-basis projection, on random symmetric matrices, matrices with repeated, zero
+// and clustered eigenvalues, and reduced neo-Hookean element Hessians.
 #include "lpg_eig9.hpp"
 
 #include <algorithm>
@@ -14,7 +15,7 @@ using lpg::eig9::sym;
 
 namespace {
 
-double g_worst_orth = 0, g_worst_rec = 0, g_worst_eig = 0;
+double g_worst_orth = 0, g_worst_rec = 0, g_worst_eig = 0, g_worst_proj = 0;
 
 void jacobi_eigenvalues(std::vector<double> A, std::vector<double>& ev) {
     for (int sweep = 0; sweep < 100; ++sweep) {
@@ -75,6 +76,37 @@ bool check(const std::vector<double>& full, const char* what) {
             orth = std::max(orth, std::fabs(s - (i == j ? 1.0 : 0.0)));
             rec = std::max(rec, std::fabs(m - full[i * N + j]));
         }
+    // The lean PSD part (eigenvalues + inverse iteration for the smaller eigen-set) against V max(d, 0) V^T.
+    {
+        double a2[45], C[45];
+        for (int r = 0; r < N; ++r)
+            for (int c = r; c < N; ++c)
+                a2[sym(r, c)] = full[r * N + c];
+        bool add_M = false;
+        struct Spill {
+            double v[lpg::eig9::kSpill];
+            void put(int i, double x) { v[i] = x; }
+            double get(int i) const { return v[i]; }
+        } spill;
+        const bool ok2 = lpg::eig9::psd_part(
+            a2, C, add_M, scale, [](double x) { return 1.0 / std::sqrt(x); }, [](double x) { return 1.0 / x; },
+            spill);
+        double proj_err = 0;
+        for (int r = 0; r < N; ++r)
+            for (int c = r; c < N; ++c) {
+                double full_proj = 0;
+                for (int k = 0; k < N; ++k)
+                    full_proj += V[r][k] * std::max(d[k], 0.0) * V[c][k];
+                const double lean = (add_M ? full[r * N + c] : 0.0) + C[sym(r, c)];
+                proj_err = std::max(proj_err, std::fabs(lean - full_proj));
+            }
+        proj_err /= scale;
+        g_worst_proj = std::max(g_worst_proj, proj_err);
+        if (!ok2 || !(proj_err <= 1e-13)) {
+            std::printf("FAIL %s: lean projection error %.3g (converged %d)\n", what, proj_err, ok2 ? 1 : 0);
+            return false;
+        }
+    }
     std::vector<double> ref, got(d, d + N);
     jacobi_eigenvalues(full, ref);
     std::sort(got.begin(), got.end());
@@ -167,7 +199,7 @@ int main() {
         ++cases;
     }
     // Repeated, zero and clustered eigenvalues: Q diag(lambda) Q^T with a random orthogonal Q.
-    for (int t = 0; t < 500; ++t) {
+    for (int t = 0; t < 1200; ++t) {
         std::vector<double> Q(81), A(81, 0.0);
         for (double& q : Q)
             q = U(rng);
@@ -188,7 +220,12 @@ int main() {
         }
         double lam[N];
         for (int i = 0; i < N; ++i)
-            lam[i] = (t % 3 == 0) ? (i < 4 ? 2.0 : -1.0) : (t % 3 == 1) ? (i < 3 ? 0.0 : U(rng)) : 1.0 + 1e-9 * U(rng);
+            lam[i] = (t % 6 == 0)   ? (i < 4 ? 2.0 : -1.0)
+                     : (t % 6 == 1) ? (i < 3 ? 0.0 : U(rng))
+                     : (t % 6 == 2) ? 1.0 + 1e-9 * U(rng)
+                     : (t % 6 == 3) ? (i < 3 ? -1.0 + 1e-12 * i : U(rng))        // a tight negative cluster
+                     : (t % 6 == 4) ? (i < 4 ? 1e-12 * U(rng) : 1.0 + U(rng))   // tiny values straddling zero
+                                    : (i < 2 ? -0.5 + 1e-15 * i : (i < 4 ? 1e-10 * U(rng) : U(rng)));
         for (int r = 0; r < N; ++r)
             for (int c = 0; c < N; ++c)
                 for (int i = 0; i < N; ++i)
@@ -230,8 +267,8 @@ int main() {
         fails += !check(neo_hookean_m(F, Dm, lam, mu), "neo-Hookean");
         ++cases;
     }
-    std::printf("%d cases, %d failures; worst orthogonality %.3g, reconstruction %.3g, eigenvalues %.3g "
-                "(relative to max |M|)\n",
-                cases, fails, g_worst_orth, g_worst_rec, g_worst_eig);
+    std::printf("%d cases, %d failures; worst orthogonality %.3g, reconstruction %.3g, eigenvalues %.3g, "
+                "lean projection %.3g (relative to max |M|)\n",
+                cases, fails, g_worst_orth, g_worst_rec, g_worst_eig, g_worst_proj);
     return fails == 0 ? 0 : 1;
 }
