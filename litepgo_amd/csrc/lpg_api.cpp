@@ -193,6 +193,7 @@ void ensure_device(lpg_tet_energy* h) {
     upload(h->d_chunk_begin, h->chunk_begin.data(), h->chunk_begin.size() * sizeof(int32_t));
     upload(h->d_seg_chunk, h->seg_chunk.data(), h->seg_chunk.size() * sizeof(int32_t));
     h->d_chunk_sum.ensure(std::max<size_t>(1, h->chunk_begin.size()) * sizeof(double));
+    h->d_seg_energy.ensure(std::max<size_t>(1, h->seg_chunk.size()) * sizeof(double));
     h->d_vertex_E.ensure(std::max<size_t>(1, s.num_vertices) * sizeof(double));
     if (h->plan.tile_vbeg.empty()) {
         // Default launch: 4 lanes per vertex, one wave per workgroup, 16 vertices per tile, segments up to
@@ -287,10 +288,15 @@ void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int w
     }
     if (h->timing)
         hip_check(hipEventRecord(h->ev[2], stream), "hipEventRecord");
+    const int segments = static_cast<int>(h->seg_chunk.size()) - 1;
+    // Per-segment energies go to `energies`; without it, a batch's segment energies go to scratch and their total
+    // to the handle's energy slot (the single-energy contract of the union).
+    double* seg_out = energies ? energies : (segments > 1 ? h->d_seg_energy.as<double>() : a.energy);
     launch_energy_reduce(a.vertex_E, h->d_chunk_begin.as<int32_t>(), static_cast<int>(h->chunk_begin.size()) - 1,
-                         h->d_chunk_sum.as<double>(), h->d_seg_chunk.as<int32_t>(),
-                         static_cast<int>(h->seg_chunk.size()) - 1, energies ? energies : a.energy, a.status,
+                         h->d_chunk_sum.as<double>(), h->d_seg_chunk.as<int32_t>(), segments, seg_out, a.status,
                          stream);
+    if (!energies && segments > 1)
+        launch_energy_total(seg_out, segments, a.energy, stream);
     if (h->timing)
         hip_check(hipEventRecord(h->ev[3], stream), "hipEventRecord");
 }

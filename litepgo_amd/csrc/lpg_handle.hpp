@@ -90,6 +90,7 @@ struct lpg_tet_energy {
     lpg::DeviceBuffer d_Xr, d_conn, d_theta, d_inc_ptr, d_inc, d_nbr_ptr, d_nbr, d_self_pos, d_row_ptr;
     lpg::DeviceBuffer d_scalars; // energy (double) + status + counter
     lpg::DeviceBuffer d_vertex_E, d_chunk_begin, d_chunk_sum, d_seg_chunk;
+    lpg::DeviceBuffer d_seg_energy; // per-mesh energies of a batch evaluated as one energy
     lpg::DeviceBuffer d_proj_slot, d_proj_list, d_proj_H;
     lpg::DeviceBuffer d_x, d_g, d_H; // staging for the host-pointer API
     lpg::DeviceBuffer d_dx;          // staging of the step direction (host-pointer feasible step)

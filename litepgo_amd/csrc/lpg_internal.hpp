@@ -176,6 +176,8 @@ void launch_hvp(const DeviceMesh& m, const double* x, const double* theta, const
 // output may be null.
 void launch_vjp(const DeviceMesh& m, const double* x, const double* theta, const double* gE, const double* w,
                 double* gx, double* gtheta, unsigned* status, hipStream_t stream);
+// out[0] = sum of seg[0..n) in a fixed order (one workgroup).
+void launch_energy_total(const double* seg, int n, double* out, hipStream_t stream);
 // Per-segment feasible steps (alphas: num_segments doubles on the device, min over the segment's tets, 1 when
 // none limits); also clears the status word.
 void launch_feasible_step(const DeviceMesh& m, const double* x, const double* dx, const int32_t* seg_tbeg,

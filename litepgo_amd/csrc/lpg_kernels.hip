@@ -805,6 +805,17 @@ __global__ __launch_bounds__(256) void k_energy_segments(const double* __restric
     }
 }
 
+// The total over segments (a batch evaluated through the single-energy entry points): one workgroup, fixed order.
+__global__ __launch_bounds__(256) void k_energy_total(const double* __restrict__ seg, int n, double* __restrict__ out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = static_cast<int>(threadIdx.x); i < n; i += 256)
+        s += seg[i];
+    const double total = tree_sum<256>(red, s);
+    if (threadIdx.x == 0)
+        out[0] = total;
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // Projection: prepass (PSD test of the 9x9 material tangent A by LDL^T) and 9x9 Jacobi eigen-clamp of M.
 // ---------------------------------------------------------------------------------------------------------------
@@ -1595,6 +1606,11 @@ void launch_energy_reduce(const double* vertex_E, const int32_t* chunk_begin, in
                            status);
         check_launch("k_energy_segments");
     }
+}
+
+void launch_energy_total(const double* seg, int n, double* out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_energy_total, dim3(1), dim3(256), 0, stream, seg, n, out);
+    check_launch("k_energy_total");
 }
 
 void launch_project_prepass(const DeviceMesh& m, const EvalArgs& a, hipStream_t stream) {

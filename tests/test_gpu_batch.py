@@ -54,6 +54,22 @@ def test_batch_equals_plain_per_mesh(torch, meshes, oracle_mod, kind):
             assert abs(v - vr) <= TOL * abs(vr) and rel(gp, gr) <= TOL and rel(Hp, Hr) <= TOL
 
 
+def test_union_through_single_energy_entry_points(torch, meshes):
+    """The batch handle used as one energy (VolumetricFemEnergy methods, host and device) reports the total of its
+    meshes' energies, and its status word stays intact (24 segment energies no longer land in the scalar slot)."""
+    batch = BatchedFemEnergy(meshes)
+    x = union_x(torch, meshes)
+    E, _, _ = batch.eval_device(x, True, "projected")
+    per_mesh = E.cpu().numpy()
+    total, g, _ = VolumetricFemEnergy.eval_device(batch, x, True, "projected")
+    assert abs(total - per_mesh.sum()) <= TOL * abs(per_mesh).sum()
+    v, gh, _ = batch.eval_values(x.cpu().numpy(), True, "exact")
+    assert abs(v - per_mesh.sum()) <= 1e-10 * abs(per_mesh).sum()
+    for _ in range(3):  # repeated calls: no stale status from overwritten scalars
+        total2, _, _ = VolumetricFemEnergy.eval_device(batch, x, False, None)
+        assert total2 == total or abs(total2 - total) <= TOL * abs(total)
+
+
 def test_graph_replays_match_eager(torch, meshes):
     batch = BatchedFemEnergy(meshes)
     x = union_x(torch, meshes)
