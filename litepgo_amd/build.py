@@ -10,7 +10,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "liblitepgo_fem.so"
-SOURCES = ["lpg_kernels.hip", "lpg_setup.cpp", "lpg_api.cpp", "lpg_batch.cpp"]
+SOURCES = ["lpg_kernels.hip", "lpg_solver.hip", "lpg_setup.cpp", "lpg_api.cpp", "lpg_batch.cpp", "lpg_newton.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops stays off (hand-written FP64 code only; packed FP32 forms are avoided on these parts).
@@ -21,9 +21,11 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-munsafe-fp-at
 
 def build(verbose: bool = False, extra: list[str] | None = None) -> Path:
     """Compile every source and link the shared library; always rebuilds."""
-    objs = []
+    from concurrent.futures import ThreadPoolExecutor
+
     build_dir = PKG / "_build"
     build_dir.mkdir(exist_ok=True)
+    cmds, objs = [], []
     for src in SOURCES:
         obj = build_dir / (src + ".o")
         cmd = [HIPCC, *COMMON, *(extra or []), "-c", str(CSRC / src), "-o", str(obj)]
@@ -32,8 +34,12 @@ def build(verbose: bool = False, extra: list[str] | None = None) -> Path:
             cmd.insert(2, "hip")
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append(cmd)
         objs.append(str(obj))
+    # Sources compile independently; at most four at a time.
+    with ThreadPoolExecutor(max_workers=min(4, len(cmds))) as pool:
+        for proc in pool.map(lambda c: subprocess.run(c, check=True), cmds):
+            pass
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs, "-lpthread"]
     if verbose:

@@ -139,12 +139,14 @@ double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter);
 // returns the first energy (the plain energy; per-mesh energies go to `energies` when given).
 double run_eval(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g, double* H,
                 double* energies, hipStream_t stream);
-// Accumulates the per-phase event times of the last eager evaluation (timing mode).
 // Feasible step of x along dx (device arrays): one bound per segment into `alphas` (device; null: the handle's
 // read-back slot), whole handle or per mesh; synchronises once and returns the first bound.
 double feasible_step_call(lpg_tet_energy* h, const double* x, const double* dx, bool per_mesh, double* alphas,
                           hipStream_t s);
+// Accumulates the per-phase event times of the last eager evaluation (timing mode).
 void collect_timing(lpg_tet_energy* h);
+// Reads back the scalars (value, status) enqueued so far on `stream`, synchronises and raises on a bad status.
+double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter);
 // The status mapping alone (status word as read back).
 void raise_status(unsigned status);
 
