@@ -9,8 +9,9 @@ gradient and the PSD-projected Hessian assembled into the CSR value array, with 
 Also reported: the assembly kernel against the HBM roofline (HIP events around the kernel on its stream) and its
 FP64 rate against the FP64 vector peak, the per-phase times, the same step with the Exact Hessian, the 256-mesh
 batched evaluation (one autotuned persistent launch sequence replayed as a hipGraph) against a per-mesh launch
-loop, the 32^3 differentiation workload's backward time against its forward time, and the oracle (C++ CPU
-restatement of the reference algorithm) timed on this host's cores.
+loop, the 32^3 differentiation workload's backward time against its forward time, the equilibrium-sensitivity
+problem (device Newton + Jacobi-PCG + adjoint on a 48^3 cantilever; 8 uneven beams with --equilibrium-batch), and
+the oracle (C++ CPU restatement of the reference algorithm) timed on this host's cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched]
 """
@@ -179,6 +180,53 @@ def bench_differentiation(steps, warmup):
             "forward_tets_per_s": T / (f_ms * 1e-3), "backward_tets_per_s": T / (b_ms * 1e-3)}
 
 
+def bench_equilibrium(batch):
+    """Equilibrium sensitivity (cantilever, x = 0 face fixed, gravity 9.8 along -z, density 1000, per-tet E
+    log-normal(ln 1e5, 0.5), nu 0.45): device Newton with Jacobi-PCG to |g|_inf < 1e-8 from rest, the same for the
+    target (a seeded perturbation of the E field), then the adjoint gradient of |x - x_target|^2 with respect to
+    the per-tet E field.  batch=False: one 48^3 beam; True: 8 beams with per-axis counts in 40..48, solved as one
+    union handle."""
+    import numpy as np
+    import torch
+
+    from litepgo_amd import BatchedFemEnergy, VolumetricFemEnergy, solver, workloads
+    from litepgo_amd import torch as lt
+
+    beams = workloads.beam_batch(8, seed=0) if batch else [workloads.beam(48, seed=0)]
+    parts = [(b.num_vertices, b.tets, b.rest, b.theta) for b in beams]
+    T = sum(b.tets.shape[0] for b in beams)
+    en = BatchedFemEnergy(parts) if batch else VolumetricFemEnergy(*parts[0])
+    fixed = solver.fixed_mask(en.num_dofs, np.concatenate([b.fixed for b in beams]))
+    lin = torch.from_numpy(np.concatenate([-b.load for b in beams])).cuda()
+    nu = torch.from_numpy(np.concatenate([b.nu for b in beams])).cuda()
+    E = torch.from_numpy(np.concatenate([b.E for b in beams])).cuda()
+    theta = lt.lame(E, nu).detach()
+    theta_t = lt.lame(torch.from_numpy(np.concatenate([b.E_target for b in beams])).cuda(), nu).detach()
+    x0 = torch.zeros(en.num_dofs, dtype=torch.float64, device="cuda")
+    opts = solver.NewtonOptions(gradient_tolerance=1e-8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = solver.newton_solve(en, fixed, x0, opts, linear=lin, theta=theta)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    res_t = solver.newton_solve(en, fixed, x0, opts, linear=lin, theta=theta_t)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    Er = E.clone().requires_grad_(True)
+    g_theta = solver.equilibrium_vjp(en, fixed, res.x, 2.0 * (res.x - res_t.x), opts, theta=theta)
+    (g_E,) = torch.autograd.grad(lt.lame(Er, nu), Er, grad_outputs=g_theta)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    ok = res.status == 0 and res_t.status == 0
+    return {"workload": ("8 cantilevers, 40..48 cells per axis" if batch else "48^3 cantilever") +
+                        ", x=0 fixed, gravity -9.8 z, density 1000, E log-normal(ln 1e5, 0.5), nu 0.45",
+            "tets": T, "dofs": en.num_dofs, "converged": ok,
+            "newton_iterations": res.iterations, "pcg_iterations": res.pcg_iterations,
+            "gradient_norm": res.gradient_norm, "solve_s": t1 - t0, "target_solve_s": t2 - t1,
+            "adjoint_s": t3 - t2, "pcg_iterations_per_s": res.pcg_iterations / (t1 - t0),
+            "objective": float(((res.x - res_t.x) ** 2).sum()), "grad_E_norm": float(g_E.norm())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +237,9 @@ def main():
     ap.add_argument("--exact-only", action="store_true", help="time only the Exact-Hessian step (profiling aid)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-mesh batched comparison")
     ap.add_argument("--no-differentiation", action="store_true", help="skip the 32^3 forward/backward timing")
+    ap.add_argument("--no-equilibrium", action="store_true", help="skip the 48^3 equilibrium-sensitivity solve")
+    ap.add_argument("--equilibrium-batch", action="store_true",
+                    help="also solve the 8-beam (40..48 cells per axis) equilibrium batch")
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a known-size kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -267,6 +318,10 @@ def main():
         result["batched"] = bench_batched(args.steps, args.warmup)
     if not args.no_differentiation:
         result["differentiation"] = bench_differentiation(args.steps, args.warmup)
+    if not args.no_equilibrium:
+        result["equilibrium"] = bench_equilibrium(batch=False)
+        if args.equilibrium_batch:
+            result["equilibrium_batch"] = bench_equilibrium(batch=True)
     if not args.no_cpu:
         nthreads = min(16, len(os.sched_getaffinity(0)))
         result["cpu_baseline"] = cpu_baseline(w, nthreads, args.cpu_repeats)

@@ -121,9 +121,13 @@ class VolumetricFemEnergy:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if not h:
+            return
+        self._h = None
+        try:
             _native.lib().lpg_tet_energy_destroy(h)
-            self._h = None
+        except (AttributeError, TypeError):
+            pass  # interpreter shutdown: module globals are already gone and the process releases the device
 
     # ---- sizes and pattern -------------------------------------------------------------------------------
     @property
