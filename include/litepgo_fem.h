@@ -57,7 +57,9 @@ size_t lpg_last_error(char* buf, size_t len);
  *   batch_size is accepted for interface parity (> 0 required); it does not change results.
  * Snapshots everything; nothing borrowed is kept.  Rest geometry is validated as VolumetricElement3D does
  * (invalid_argument on det(Dm) <= 0); theta is validated at evaluation, as NeoHookean does.
- * Only Fp == NULL (or all-identity Fp) is supported on the device; anything else is LPG_INVALID_ARGUMENT. */
+ * A plastic part Fp != I enters as MultiplicativeKinematics does (multiplicative_kinematics.cpp:9-51): Fe = F Fp^-1,
+ * W = det(Fp) psi(Fe); a non-finite Fp (LPG_INVALID_ARGUMENT) or one with det(Fp) <= 0 or a non-finite inverse
+ * (LPG_DOMAIN_ERROR) is accepted here and fails every evaluation, as the reference fails when evaluating. */
 lpg_status lpg_tet_energy_create(int32_t num_vertices, int32_t num_tets, const int32_t* tets,
                                  const double* rest_positions, const double* theta, const double* Fp,
                                  int32_t batch_size, lpg_tet_energy** out);

@@ -84,6 +84,12 @@ def lib():
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
                                "g.build()'` (there is no CPU fallback)")
+        # One HIP runtime per process: when PyTorch is present, its runtime is loaded first so that the library's
+        # libamdhip64 dependency binds to it (by soname) and device pointers of torch tensors are the library's.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

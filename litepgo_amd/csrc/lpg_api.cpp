@@ -79,6 +79,7 @@ lpg::DeviceMesh lpg_tet_energy::mesh() const {
     m.nbr = d_nbr.as<int32_t>();
     m.self_pos = d_self_pos.as<int32_t>();
     m.row_ptr = d_row_ptr.as<int32_t>();
+    m.fp = setup.fp.empty() ? nullptr : d_fp.as<double>();
     m.num_vertices = setup.num_vertices;
     m.num_tets = setup.num_tets;
     m.wide = setup.wide;
@@ -89,6 +90,7 @@ lpg_tet_energy::~lpg_tet_energy() {
     graph.reset();
     for (lpg::DeviceBuffer* b : {&d_Xr, &d_conn, &d_theta, &d_inc_ptr, &d_inc, &d_nbr_ptr, &d_nbr, &d_self_pos, &d_row_ptr, &d_scalars, &d_vertex_E, &d_chunk_begin,
                                  &d_chunk_sum, &d_seg_chunk, &d_proj_slot, &d_proj_list, &d_proj_H, &d_x, &d_g, &d_H,
+                                 &d_dx, &d_seg_tbeg, &d_seg_all, &d_seg_energy, &d_fp,
                                  &plan.d_tile_vbeg, &plan.d_normal_ids, &plan.d_oversized_ids})
         b->release();
     if (pinned)
@@ -140,6 +142,9 @@ void set_plan(lpg_tet_energy* h, const LaunchConfig& lc, int vertices_per_tile, 
 // Device state is created on the first call that needs the GPU, so construction, the pattern and the slot map
 // work on hosts without one.
 void ensure_device(lpg_tet_energy* h) {
+    // A rejected plastic part fails every evaluation (the reference raises it per evaluated element).
+    if (h->setup.fp_error)
+        fail(h->setup.fp_error, h->setup.fp_error_msg);
     if (h->device_ready) {
         hip_check(hipSetDevice(h->device), "hipSetDevice");
         return;
@@ -158,6 +163,8 @@ void ensure_device(lpg_tet_energy* h) {
     upload(h->d_nbr, s.nbr.data(), s.nbr.size() * sizeof(int32_t));
     upload(h->d_row_ptr, s.row_ptr.data(), s.row_ptr.size() * sizeof(int32_t));
     upload(h->d_self_pos, s.self_pos.data(), s.self_pos.size() * sizeof(int32_t));
+    if (!s.fp.empty())
+        upload(h->d_fp, s.fp.data(), s.fp.size() * sizeof(double));
     // Device incidence records (IncNarrow / IncWide): slot a of v in the tet, and the column positions of the tet's
     // vertices at slots a^1, a^2, a^3.
     const size_t ninc = s.inc_tet.size();

@@ -71,6 +71,12 @@ struct MeshSetup {
     std::vector<int32_t> conn;       // T x 4 vertex ids
     std::vector<double> Xr;          // V x 4 rest positions (x, y, z, 0)
     std::vector<double> theta;       // T x 2 (lambda, mu) snapshot
+    // Plastic parts, empty when every Fp is the identity: per tet row-major Fp^-1 and det Fp (10 doubles).  An Fp
+    // that MultiplicativeKinematics rejects (multiplicative_kinematics.cpp:9-21) is recorded here and raised by
+    // every evaluation, as the reference raises it per evaluated element.
+    std::vector<double> fp;
+    int fp_error = 0;
+    std::string fp_error_msg;
 };
 
 // Builds MeshSetup; throws Error(kInvalid, ...) with the reference's messages on invalid input.
@@ -107,6 +113,7 @@ struct DeviceMesh {
     const int32_t* nbr = nullptr;
     const int32_t* self_pos = nullptr;
     const int32_t* row_ptr = nullptr;
+    const double* fp = nullptr; // T x 10 plastic parts (row-major Fp^-1, det Fp), null when every Fp = I
     int32_t num_vertices = 0;
     int32_t num_tets = 0;
     bool wide = false;

@@ -57,6 +57,8 @@ struct TetIn {
     double X[4][3];
     double u[4][3];
     double lam, mu;
+    // Plastic part of the tet (row-major Fp^-1, then det Fp: 10 doubles), or null for Fp = I.
+    const double* fp = nullptr;
 };
 
 __device__ __forceinline__ void load_vertices(const double* __restrict__ Xr, const double* __restrict__ x,
@@ -85,7 +87,9 @@ __device__ __forceinline__ void load_theta(const double* __restrict__ snapshot, 
 // Element-wise kernels: connectivity (int4) and theta (double2) of tet e are read coalesced, vertex data gathered.
 __device__ __forceinline__ void load_tet(const double* __restrict__ Xr, const int32_t* __restrict__ conn,
                                          const double* __restrict__ snapshot, const double* __restrict__ x,
-                                         const double* __restrict__ override_, int e, int ids[4], TetIn& t) {
+                                         const double* __restrict__ override_, int e, int ids[4], TetIn& t,
+                                         const double* __restrict__ fp = nullptr) {
+    t.fp = fp ? fp + 10 * static_cast<size_t>(e) : nullptr;
     const int4 c = reinterpret_cast<const int4*>(conn)[e];
     ids[0] = c.x;
     ids[1] = c.y;
@@ -97,9 +101,14 @@ __device__ __forceinline__ void load_tet(const double* __restrict__ Xr, const in
 
 // Rest geometry: Dm = [X1-X0 | X2-X0 | X3-X0], Dm^-1 = cof(Dm)^T / det, w = det / 6.  The host applies the same
 // operation sequence when it validates the mesh (det > 0, finite Dm^-1), so a tet accepted there is valid here.
+// With a plastic part Fp (multiplicative_kinematics.cpp:9-51), Fe = F Fp^-1 = Fp^-1 + Du (Dm^-1 Fp^-1) and
+// W = det(Fp) psi(Fe): the element is a plain neo-Hookean tet with rest edges Fp Dm, so Di holds Dm^-1 Fp^-1, w is
+// scaled by det Fp and F0 = Fp^-1 replaces the identity in F.
 struct Geo {
-    double Di[9]; // Dm^-1 row-major
+    double Di[9]; // Dm^-1 (Fp^-1) row-major
     double w;
+    bool plastic = false;
+    double F0[9];  // Fp^-1 row-major (plastic only)
 };
 
 __device__ __forceinline__ void rest_geometry(const TetIn& t, Geo& g) {
@@ -125,16 +134,34 @@ __device__ __forceinline__ void rest_geometry(const TetIn& t, Geo& g) {
         for (int j = 0; j < 3; ++j)
             g.Di[3 * i + j] = C[3 * j + i] * inv;
     g.w = det * (1.0 / 6.0);
+    g.plastic = t.fp != nullptr;
+    if (g.plastic) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+            g.F0[q] = t.fp[q];
+        double Dp[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                Dp[3 * i + j] = dot3(g.Di[3 * i], g.Di[3 * i + 1], g.Di[3 * i + 2], g.F0[j], g.F0[3 + j], g.F0[6 + j]);
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+            g.Di[q] = Dp[q];
+        g.w *= t.fp[9];
+    }
 }
 
-// Deformation gradient and its inverse transpose: F = I + Du Dm^-1, a = F^-T = cof(F) / det F.
+// Deformation gradient and its inverse transpose: F = I + Du Dm^-1 (Fp^-1 + Du Dm^-1 Fp^-1 with a plastic part),
+// a = F^-T = cof(F) / det F.
 struct Kin {
     double F[9]; // row-major
     double a[9]; // row-major, a(i, j) = F^-T(i, j)
     double J, L;
 };
 
-__device__ __forceinline__ void kinematics(const TetIn& t, const double Di[9], Kin& k) {
+__device__ __forceinline__ void kinematics(const TetIn& t, const Geo& geo, Kin& k) {
+    const double* Di = geo.Di;
     double Du[9]; // Du(i, c) = (u_{c+1} - u_0)_i
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -145,7 +172,7 @@ __device__ __forceinline__ void kinematics(const TetIn& t, const double Di[9], K
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            double s = (i == j) ? 1.0 : 0.0;
+            double s = geo.plastic ? geo.F0[3 * i + j] : ((i == j) ? 1.0 : 0.0);
 #pragma unroll
             for (int c = 0; c < 3; ++c)
                 s = fma(Du[3 * i + c], Di[3 * c + j], s);
@@ -274,6 +301,7 @@ struct AssembleParams {
     unsigned* status;
     const int32_t* proj_slot;
     const double* proj_H;
+    const double* fp = nullptr; // plastic parts (10 per tet) or null
 };
 
 // The exact blocks H_0j (j = 1..3) and the 6 distinct entries of the symmetric H_00 of one incidence (KIND >= 0),
@@ -300,7 +328,6 @@ __device__ __forceinline__ void incidence_values(const TetIn& t, int a, double o
         }
     const double det_m = dot3(Dm[0], Dm[1], Dm[2], Cm[0], Cm[1], Cm[2]);
     const double inv_m = recip(det_m);
-    const double w = det_m * (1.0 / 6.0);
     double G[4][3]; // rows of dN/dxi Dm^-1
 #pragma unroll
     for (int b = 1; b < 4; ++b)
@@ -310,6 +337,20 @@ __device__ __forceinline__ void incidence_values(const TetIn& t, int a, double o
 #pragma unroll
     for (int j = 0; j < 3; ++j)
         G[0][j] = -(G[1][j] + G[2][j] + G[3][j]);
+    // Plastic part: G <- G Fp^-1, w <- w det Fp, J = det Ds / (det Dm det Fp), F = Fp^-1 + Du G (below).
+    double w = det_m * (1.0 / 6.0);
+    double inv_mp = inv_m;
+    if (t.fp) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const double g0 = G[b][0], g1 = G[b][1], g2 = G[b][2];
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                G[b][j] = dot3(g0, g1, g2, t.fp[j], t.fp[3 + j], t.fp[6 + j]);
+        }
+        w *= t.fp[9];
+        inv_mp = inv_m * recip(t.fp[9]);
+    }
     // Current geometry.
     double Du[9], Ds[9], Cs[9];
 #pragma unroll
@@ -328,7 +369,7 @@ __device__ __forceinline__ void incidence_values(const TetIn& t, int a, double o
         }
     const double det_s = dot3(Ds[0], Ds[1], Ds[2], Cs[0], Cs[1], Cs[2]);
     const double inv_s = recip(det_s);
-    const double J = det_s * inv_m;
+    const double J = det_s * inv_mp;
     const double L = log(J);
     if (!(det_m > 0.0 && J > 0.0 && isfinite(J) && isfinite(L)))
         flags |= kStatusDetF;
@@ -345,12 +386,21 @@ __device__ __forceinline__ void incidence_values(const TetIn& t, int a, double o
         q[0][j] = -(q[1][j] + q[2][j] + q[3][j]);
     // F = I + Du Dm^-1.
     double F[9];
+    if (!t.fp) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
-            F[3 * i + j] =
-                fma(Du[3 * i + 2], G[3][j], fma(Du[3 * i + 1], G[2][j], fma(Du[3 * i], G[1][j], i == j ? 1.0 : 0.0)));
+            for (int j = 0; j < 3; ++j)
+                F[3 * i + j] = fma(Du[3 * i + 2], G[3][j],
+                                   fma(Du[3 * i + 1], G[2][j], fma(Du[3 * i], G[1][j], i == j ? 1.0 : 0.0)));
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                F[3 * i + j] = fma(Du[3 * i + 2], G[3][j],
+                                   fma(Du[3 * i + 1], G[2][j], fma(Du[3 * i], G[1][j], t.fp[3 * i + j])));
+    }
     const double p_ = fma(t.lam, L, -t.mu);
     // Gradient block w (mu F g_0 + p q_0).
 #pragma unroll
@@ -514,6 +564,7 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
                     }
                 t.lam = d.lam;
                 t.mu = d.mu;
+                t.fp = p.fp ? p.fp + 10 * static_cast<size_t>(r.tet) : nullptr;
                 double gc[3], ec = 0.0, dg[6];
                 incidence_values<KIND>(t, IncT<WIDE>::slot(r), off, dg, gc, ec, flags);
                 gs0 += gc[0];
@@ -831,6 +882,7 @@ struct ProjectParams {
     int32_t* proj_slot;
     int32_t* proj_list;
     double* proj_H;
+    const double* fp = nullptr; // plastic parts (10 per tet) or null
 };
 
 __device__ __forceinline__ int sym_index(int r, int c) {
@@ -848,11 +900,11 @@ __global__ __launch_bounds__(kPrepassThreads) void k_project_prepass(ProjectPara
     if (e < p.num_tets) {
         int ids[4];
         TetIn t;
-        load_tet(p.Xr, p.conn, p.theta_snap, p.x, p.theta, e, ids, t);
+        load_tet(p.Xr, p.conn, p.theta_snap, p.x, p.theta, e, ids, t, p.fp);
         Geo geo;
         rest_geometry(t, geo);
         Kin k;
-        kinematics(t, geo.Di, k);
+        kinematics(t, geo, k);
         if (kin_ok(k) && theta_ok(t.lam, t.mu)) {
             const double pr = fma(t.lam, k.L, -t.mu);
             // A(i+3j, m+3l) = mu d_im d_jl + lambda a_ij a_ml - p a_il a_mj, packed upper (45).
@@ -922,11 +974,11 @@ __global__ __launch_bounds__(kPrepassThreads) void k_project_prepass(ProjectPara
 __device__ __forceinline__ void reduced_hessian(const ProjectParams& p, int e, double (&M)[45]) {
     int ids[4];
     TetIn t;
-    load_tet(p.Xr, p.conn, p.theta_snap, p.x, p.theta, e, ids, t);
+    load_tet(p.Xr, p.conn, p.theta_snap, p.x, p.theta, e, ids, t, p.fp);
     Geo geo;
     rest_geometry(t, geo);
     Kin k;
-    kinematics(t, geo.Di, k);
+    kinematics(t, geo, k);
     double G[4][3];
     shape_gradients(geo.Di, G);
     const double pr = fma(t.lam, k.L, -t.mu);
@@ -1059,6 +1111,7 @@ struct ElemParams {
     double* out;          // T x 2
     unsigned* status;
     int num_tets;
+    const double* fp = nullptr; // plastic parts (10 per tet) or null
 };
 
 // Per-tet theta cotangent: out_e = s dE/dtheta_e + (d g_e / d theta_e)^T w, with
@@ -1073,11 +1126,11 @@ template <bool HAS_S, bool HAS_W> __global__ __launch_bounds__(256) void k_theta
         return;
     int ids[4];
     TetIn t;
-    load_tet(p.Xr, p.conn, p.theta_snap, p.x, p.theta, e, ids, t);
+    load_tet(p.Xr, p.conn, p.theta_snap, p.x, p.theta, e, ids, t, p.fp);
     Geo geo;
     rest_geometry(t, geo);
     Kin k;
-    kinematics(t, geo.Di, k);
+    kinematics(t, geo, k);
     unsigned flags = 0;
     if (!kin_ok(k))
         flags |= kStatusDetF;
@@ -1142,6 +1195,7 @@ struct GatherParams {
     double* out;     // num_dofs
     unsigned* status;
     int num_vertices;
+    const double* fp = nullptr; // plastic parts (10 per tet) or null
 };
 
 constexpr int kGatherLanes = 8;
@@ -1165,12 +1219,12 @@ template <int MODE> __global__ __launch_bounds__(kGatherThreads) void k_vertex_g
                                                          static_cast<size_t>(kk) * p.inc_stride);
         int ids[4];
         TetIn t;
-        load_tet(p.Xr, p.conn, p.theta_snap, p.x, p.theta, e, ids, t);
+        load_tet(p.Xr, p.conn, p.theta_snap, p.x, p.theta, e, ids, t, p.fp);
         Geo geo;
         rest_geometry(t, geo);
         const int a = (ids[0] == v) ? 0 : (ids[1] == v) ? 1 : (ids[2] == v) ? 2 : 3;
         Kin k;
-        kinematics(t, geo.Di, k);
+        kinematics(t, geo, k);
         if (!kin_ok(k))
             flags |= kStatusDetF;
         if (!theta_ok(t.lam, t.mu))
@@ -1406,7 +1460,7 @@ __global__ __launch_bounds__(256) void k_feasible_step(StepParams p) {
         Geo geo;
         rest_geometry(t, geo);
         Kin k;
-        kinematics(t, geo.Di, k);
+        kinematics(t, geo, k);
         if (!kin_ok(k))
             flags |= kStatusDetF;
         if (!theta_ok(t.lam, t.mu))
@@ -1577,6 +1631,7 @@ void launch_assemble(const DeviceMesh& m, const EvalArgs& a, const LaunchConfig&
     p.status = a.status;
     p.proj_slot = a.proj_slot;
     p.proj_H = a.proj_H;
+    p.fp = m.fp;
     const size_t bytes = assemble_lds_bytes(lc, caps, global_acc, a.kind);
     if (bytes > assemble_lds_limit(lc))
         fail(kLogic, "launch_assemble: staged tile data exceeds the LDS of a workgroup");
@@ -1617,6 +1672,7 @@ void launch_project_prepass(const DeviceMesh& m, const EvalArgs& a, hipStream_t 
     if (m.num_tets == 0)
         return;
     ProjectParams p{m.Xr, m.conn, m.theta, a.x, a.theta, m.num_tets, a.status, a.counter, a.proj_slot, a.proj_list, a.proj_H};
+    p.fp = m.fp;
     hipLaunchKernelGGL(k_project_prepass, dim3((m.num_tets + kPrepassThreads - 1) / kPrepassThreads),
                        dim3(kPrepassThreads), 0, stream, p);
     check_launch("k_project_prepass");
@@ -1626,6 +1682,7 @@ void launch_project_jacobi(const DeviceMesh& m, const EvalArgs& a, hipStream_t s
     if (m.num_tets == 0)
         return;
     ProjectParams p{m.Xr, m.conn, m.theta, a.x, a.theta, m.num_tets, a.status, a.counter, a.proj_slot, a.proj_list, a.proj_H};
+    p.fp = m.fp;
     // Enough single-wave workgroups for two waves per SIMD on 256 CUs; the list length is only known on the device,
     // so surplus workgroups exit at once and fewer elements than workgroups leave the rest idle.
     const int blocks = std::min(8192, (m.num_tets + 63) / 64);
@@ -1639,6 +1696,7 @@ static void launch_theta_vjp(const DeviceMesh& m, const double* x, const double*
     if (m.num_tets == 0)
         return;
     ElemParams p{m.Xr, m.conn, m.theta, x, theta, w, scale, scale_val, out, status, m.num_tets};
+    p.fp = m.fp;
     const dim3 grid((m.num_tets + 255) / 256), block(256);
     if (has_s && w)
         hipLaunchKernelGGL((k_theta_vjp<true, true>), grid, block, 0, stream, p);
@@ -1676,6 +1734,7 @@ static void launch_gather(int mode, const DeviceMesh& m, const double* x, const 
                    out,
                    status,
                    m.num_vertices};
+    p.fp = m.fp;
     const long long threads = static_cast<long long>(m.num_vertices) * kGatherLanes;
     const dim3 grid(static_cast<unsigned>((threads + kGatherThreads - 1) / kGatherThreads)), block(kGatherThreads);
     if (mode == 0)

@@ -71,11 +71,55 @@ void build_mesh_setup(MeshSetup& s, int32_t nv, int32_t nt, const int32_t* tets,
         if (v[0] == v[1] || v[1] == v[2] || v[2] == v[3])
             fail(kInvalid, "MeshTopology cell " + std::to_string(e) + " must contain distinct vertices");
     }
-    if (Fp) {
-        for (int64_t e = 0; e < nt; ++e)
+    s.fp.clear();
+    s.fp_error = 0;
+    s.fp_error_msg.clear();
+    bool identity = true;
+    if (Fp)
+        for (int64_t e = 0; e < nt && identity; ++e)
             for (int k = 0; k < 9; ++k)
                 if (Fp[9 * e + k] != (k % 4 == 0 ? 1.0 : 0.0))
-                    fail(kInvalid, "lpg_tet_energy_create: only Fp = I is supported by the device path");
+                    identity = false;
+    if (!identity) {
+        // Fp column-major (Fp(i, j) = Fp[9 e + i + 3 j]); Fp^-1 = adj(Fp) / det Fp, stored row-major.
+        s.fp.assign(10 * static_cast<size_t>(nt), 0.0);
+        for (int64_t e = 0; e < nt; ++e) {
+            double A[9]; // row-major
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    A[3 * i + j] = Fp[9 * e + i + 3 * j];
+            bool finite = true;
+            for (double a : A)
+                finite = finite && std::isfinite(a);
+            if (!finite) {
+                if (!s.fp_error) {
+                    s.fp_error = kInvalid;
+                    s.fp_error_msg = "MultiplicativeKinematics requires finite F and Fp";
+                }
+                continue;
+            }
+            double C[9];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+                    C[3 * i + j] = A[3 * i1 + j1] * A[3 * i2 + j2] - A[3 * i1 + j2] * A[3 * i2 + j1];
+                }
+            const double det = A[0] * C[0] + A[1] * C[1] + A[2] * C[2];
+            double* out = &s.fp[10 * e];
+            bool ok = std::isfinite(det) && det > 0.0;
+            for (int i = 0; i < 3 && ok; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    out[3 * i + j] = C[3 * j + i] / det;
+                    ok = ok && std::isfinite(out[3 * i + j]);
+                }
+            out[9] = det;
+            if (!ok && !s.fp_error) {
+                s.fp_error = kDomain;
+                s.fp_error_msg = (std::isfinite(det) && det > 0.0)
+                                     ? "MultiplicativeKinematics: non-finite inverse or elastic gradient"
+                                     : "MultiplicativeKinematics requires finite det(Fp) > 0";
+            }
+        }
     }
     s.num_vertices = nv;
     s.num_tets = nt;

@@ -8,7 +8,7 @@ take and return HIP float64 torch tensors and never leave the GPU.
 
 Construction takes the mesh as arrays (tets (T, 4), rest positions (V, 3), per-element theta (T, 2) = (lambda, mu))
 instead of the reference's per-element Response objects (fem/assembly/volumetric_fem_energy.hpp:29-33); the
-values they sample are the same: rest positions interpolated by TetP1Basis, constant Fp = I and constant theta.
+values they sample are the same: rest positions interpolated by TetP1Basis, and Fp and theta constant per element.
 """
 from __future__ import annotations
 
@@ -91,7 +91,8 @@ class VolumetricFemEnergy:
     """VolumetricFemEnergy[NeoHookean, TetP1Basis, TetCentroidQuadrature] on the device.
 
     num_vertices: V; tets: (T, 4) vertex ids; rest_positions: (V, 3); theta: (T, 2) (lambda, mu) or (2,) shared;
-    Fp: None or (T, 3, 3) identity (the device path supports Fp = I only); batch_size > 0 as in the reference.
+    Fp: None (identity) or (T, 3, 3) plastic parts (Fe = F Fp^-1, W = det(Fp) psi(Fe)); batch_size > 0 as in the
+    reference.
     """
 
     def __init__(self, num_vertices, tets, rest_positions, theta, Fp=None, batch_size=256):

@@ -1,0 +1,98 @@
+"""Plastic parts (Fp != I) on the MI355X against the oracle, which restates MultiplicativeKinematics
+(multiplicative_kinematics.cpp:9-51: Fe = F Fp^-1, W = det(Fp) psi(Fe), P and A pulled back) in its generic dense
+route: energy, gradient, Exact and Projected Hessians, the differentiation products and the feasible step, each
+within 1e-12 relative max-norm; and the evaluation-time errors of a rejected Fp."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from litepgo_amd import VolumetricFemEnergy, kuhn_grid, lame_parameters
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+@pytest.fixture(scope="module", autouse=True)
+def torch():
+    # Initialise torch's HIP context before the library's first call in this process.
+    import torch
+
+    assert torch.cuda.is_available()
+    return torch
+
+
+@pytest.fixture(scope="module")
+def case(oracle_mod):
+    X, tets = kuhn_grid(6)
+    rng = np.random.default_rng(4)
+    T = tets.shape[0]
+    E = np.exp(rng.normal(np.log(1e5), 0.3, size=T))
+    lam, mu = lame_parameters(E, 0.4)
+    theta = np.stack([lam, mu], axis=1)
+    Fp = np.eye(3)[None] + 0.15 * rng.normal(size=(T, 3, 3))  # generic plastic parts, det > 0
+    assert np.all(np.linalg.det(Fp) > 0)
+    u = 0.03 * rng.normal(size=X.size) / 6
+    dev = VolumetricFemEnergy(len(X), tets, X, theta, Fp=Fp)
+    ref = oracle_mod.OracleEnergy(len(X), tets, X, theta, Fp=Fp)
+    return X, tets, theta, Fp, u, dev, ref
+
+
+@pytest.mark.parametrize("kind", ["exact", "projected"])
+def test_energy_gradient_hessian(case, kind):
+    X, tets, theta, Fp, u, dev, ref = case
+    v, g, H = dev.eval_values(u, True, kind)
+    vr, gr, Hr = ref.eval(u, True, kind)
+    print(f"{kind}: dE {abs(v - vr) / abs(vr):.2e} dg {rel(g, gr):.2e} dH {rel(H, Hr):.2e}", flush=True)
+    assert abs(v - vr) <= TOL * abs(vr)
+    assert rel(g, gr) <= TOL
+    assert rel(H, Hr) <= TOL
+    if kind == "projected":
+        assert dev.last_indefinite_count > 0  # the plastic parts make some tangents indefinite
+
+
+def test_derivative_products(torch, case):
+    X, tets, theta, Fp, u, dev, ref = case
+    x = torch.from_numpy(u).cuda()
+    rng = np.random.default_rng(5)
+    assert rel(dev.dE_dtheta_device(x).cpu().numpy(), ref.dE_dtheta(u)) <= TOL
+    w = rng.normal(size=u.size)
+    assert rel(dev.force_vjp_theta_device(x, torch.from_numpy(w).cuda()).cpu().numpy(),
+               ref.force_theta_vjp(u, w)) <= TOL
+    dth = rng.normal(size=theta.shape)
+    assert rel(dev.force_jvp_theta_device(x, torch.from_numpy(dth).cuda()).cpu().numpy(),
+               ref.force_theta_jvp(u, dth)) <= TOL
+    _, _, Hr = ref.eval(u, False, "exact")
+    outer, inner = ref.pattern()
+    Hm = sp.csc_matrix((Hr, inner, outer), shape=(ref.num_dofs, ref.num_dofs))
+    vv = rng.normal(size=u.size)
+    assert rel(dev.hvp_device(x, torch.from_numpy(vv).cuda()).cpu().numpy(), Hm @ vv) <= TOL
+
+
+def test_feasible_step(case):
+    X, tets, theta, Fp, u, dev, ref = case
+    rng = np.random.default_rng(6)
+    for scale in (0.01, 0.2, 2.0):
+        dx = rng.normal(scale=scale, size=u.size)
+        a, r = dev.max_feasible_step(u, dx), ref.max_feasible_step(u, dx)
+        assert (a == r == 1.0) or abs(a - r) <= TOL * r
+
+
+def test_rejected_plastic_parts_fail_evaluations(torch):
+    X, tets = kuhn_grid(2)
+    T = tets.shape[0]
+    lam, mu = lame_parameters(1e5, 0.4)
+    theta = np.tile([lam, mu], (T, 1))
+    u = np.zeros(X.size)
+    for Fp_bad, kind in ((-1.0, "det"), (np.nan, "finite")):
+        Fp = np.stack([np.eye(3)] * T)
+        Fp[3] = Fp_bad * np.eye(3) if kind == "det" else np.full((3, 3), np.nan)
+        dev = VolumetricFemEnergy(len(X), tets, X, theta, Fp=Fp)
+        with pytest.raises(ValueError):
+            dev.eval(u, True, "exact")
+        with pytest.raises(ValueError):
+            dev.max_feasible_step(u, u)

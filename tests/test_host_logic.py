@@ -60,9 +60,9 @@ def test_construction_errors_follow_the_reference():
         VolumetricFemEnergy(5, np.array([[0, 1, 2, 7]]), X, th[:1])
     with pytest.raises(ValueError, match="batch size"):
         VolumetricFemEnergy(5, T, X, th, batch_size=0)
-    with pytest.raises(ValueError, match="Fp = I"):
-        VolumetricFemEnergy(5, T, X, th, Fp=np.stack([2 * np.eye(3)] * 2))
-    # identity Fp is accepted; invalid theta is accepted at construction (NeoHookean checks it when evaluating)
+    # any Fp is accepted at construction (MultiplicativeKinematics checks it when evaluating), as is invalid theta
+    VolumetricFemEnergy(5, T, X, th, Fp=np.stack([2 * np.eye(3)] * 2))
+    VolumetricFemEnergy(5, T, X, th, Fp=np.stack([-np.eye(3)] * 2))
     VolumetricFemEnergy(5, T, X, th, Fp=np.stack([np.eye(3)] * 2))
     VolumetricFemEnergy(5, T, X, np.array([[2.0, 3.0], [-1.0, 3.0]]))
 
