@@ -487,5 +487,54 @@ LPG_EIG_FN bool psd_part(double a[45], double C[45], bool& add_M, double scale, 
     return ok;
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// Definiteness certificate of the neo-Hookean tangent A(i+3j, m+3l) = mu d_im d_jl + lambda a_ij a_ml - p a_il a_mj
+// (a = F^-T, p = lambda log J - mu; neo_hookean.cpp:43-60) without forming A.  In the singular-value frame
+// F = U diag(sigma) V^T the 9x9 form splits into
+//   * three 2x2 blocks [[mu, -p s_ij], [-p s_ij, mu]] with s_ij = 1 / (sigma_i sigma_j), i < j: positive definite
+//     iff mu sigma_i sigma_j > |p| for every pair, i.e. (min pair product = J / sigma_max)
+//     mu^2 J^2 > p^2 sigma_max^2, i.e. the 3x3 matrix X = mu^2 J^2 I - p^2 C (C = F^T F) is positive definite;
+//   * the 3x3 block B = diag(mu - p / c_i) + lambda s s^T (c_i = sigma_i^2, s_i = 1 / sigma_i).  For p <= 0 it is
+//     positive definite.  For p > 0 and the pair condition strict, c_mid >= sigma_min sigma_mid > p / mu, so at most
+//     one diagonal entry is negative and, by interlacing, B > 0 iff det B > 0; det B = N / I3 with
+//     N = mu^3 I3 - mu^2 p I2 + mu p^2 I1 - p^3 + lambda (mu^2 I2 - 2 mu p I1 + 3 p^2)
+//     (I1, I2, I3: the invariants of C; the determinant lemma det(D + lambda s s^T) = det D + lambda s^T adj(D) s).
+// Returns true only when both hold with a relative margin of kTangentMargin, so a certified tangent is positive
+// definite beyond rounding; false means indefinite or too close to singular to tell (the caller then runs the
+// full eigen-analysis).  F row-major (F[3i+j] = F_ij), J = det F.
+constexpr double kTangentMargin = 1e-10;
+
+LPG_EIG_FN bool tangent_certified_pd(const double F[9], double J, double lam, double mu, double p) {
+    double C[9]; // C = F^T F, C[3j+k] = sum_i F_ij F_ik
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = j; k < 3; ++k) {
+            const double v = fma(F[j], F[k], fma(F[3 + j], F[3 + k], F[6 + j] * F[6 + k]));
+            C[3 * j + k] = v;
+            C[3 * k + j] = v;
+        }
+    const double p2 = p * p;
+    const double t = (1.0 - kTangentMargin) * (mu * J) * (mu * J);
+    const double x00 = fma(-p2, C[0], t), x11 = fma(-p2, C[4], t), x22 = fma(-p2, C[8], t);
+    const double x01 = -p2 * C[1], x02 = -p2 * C[2], x12 = -p2 * C[5];
+    const double m2 = fma(x00, x11, -(x01 * x01));
+    const double m3 = fma(x00, fma(x11, x22, -(x12 * x12)), fma(-x01, fma(x01, x22, -(x12 * x02)),
+                                                                 x02 * fma(x01, x12, -(x11 * x02))));
+    if (!(x00 > 0.0 && m2 > 0.0 && m3 > 0.0))
+        return false;
+    if (p <= 0.0)
+        return true;
+    const double I1 = C[0] + C[4] + C[8];
+    const double I2 = fma(C[0], C[4], fma(C[0], C[8], fma(C[4], C[8], -fma(C[1], C[1], fma(C[2], C[2], C[5] * C[5])))));
+    const double I3 = J * J;
+    const double mu2 = mu * mu, mu3 = mu2 * mu, p3 = p2 * p;
+    const double t1 = mu3 * I3, t2 = mu2 * p * I2, t3 = mu * p2 * I1;
+    const double t4 = lam * mu2 * I2, t5 = 2.0 * lam * mu * p * I1, t6 = 3.0 * lam * p2;
+    const double N3 = ((t1 - t2) + (t3 - p3)) + ((t4 - t5) + t6);
+    const double mag = ((fabs(t1) + fabs(t2)) + (fabs(t3) + p3)) + ((fabs(t4) + fabs(t5)) + t6);
+    return N3 > kTangentMargin * mag;
+}
+
 } // namespace eig9
 } // namespace lpg

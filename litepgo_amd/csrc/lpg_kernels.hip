@@ -21,8 +21,8 @@
 // The Projected Hessian clamps the spectrum of the 12x12 element matrix (mesh_energy.cpp:16-27).  With the
 // orthonormal Helmert basis P (4x3) of the complement of (1,1,1,1), H_e = Q M Q^T with Q = P (x) I3 and the 9x9
 // M = w (C (x) I3) A (C (x) I3)^T, C = P^T G; hence proj(H_e) = Q proj(M) Q^T, and by Sylvester's law M is PSD
-// exactly when A is.  A prepass tests A with an LDL^T factorization; only elements with an indefinite A get a
-// 9x9 Jacobi eigen-clamp of M, and the assembly reconstructs their blocks from proj(M).
+// exactly when A is.  A prepass certifies A positive definite in closed form (eig9::tangent_certified_pd); only the
+// other elements get the 9x9 eigen-clamp of M, and the assembly reconstructs their blocks from proj(M).
 #include "lpg_internal.hpp"
 #include "lpg_eig9.hpp"
 
@@ -868,7 +868,8 @@ __global__ __launch_bounds__(256) void k_energy_total(const double* __restrict__
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// Projection: prepass (PSD test of the 9x9 material tangent A by LDL^T) and 9x9 Jacobi eigen-clamp of M.
+// Projection: prepass (closed-form definiteness certificate of the 9x9 material tangent A) and the 9x9 eigen-clamp
+// of M.
 // ---------------------------------------------------------------------------------------------------------------
 struct ProjectParams {
     const double* Xr;
@@ -905,41 +906,10 @@ __global__ __launch_bounds__(kPrepassThreads) void k_project_prepass(ProjectPara
         rest_geometry(t, geo);
         Kin k;
         kinematics(t, geo, k);
-        if (kin_ok(k) && theta_ok(t.lam, t.mu)) {
-            const double pr = fma(t.lam, k.L, -t.mu);
-            // A(i+3j, m+3l) = mu d_im d_jl + lambda a_ij a_ml - p a_il a_mj, packed upper (45).
-            double A[45];
-#pragma unroll
-            for (int r = 0; r < 9; ++r)
-#pragma unroll
-                for (int c = r; c < 9; ++c) {
-                    const int i = r % 3, j = r / 3, m = c % 3, l = c / 3;
-                    double val = fma(t.lam * k.a[3 * i + j], k.a[3 * m + l], -(pr * k.a[3 * i + l] * k.a[3 * m + j]));
-                    if (i == m && j == l)
-                        val += t.mu;
-                    A[sym_index(r, c)] = val;
-                }
-            double scale = 0.0;
-#pragma unroll
-            for (int r = 0; r < 9; ++r)
-                scale = fmax(scale, fabs(A[sym_index(r, r)]));
-            // In-place LDL^T; positive definite iff every pivot is positive.
-            bool pd = true;
-#pragma unroll
-            for (int j = 0; j < 9; ++j) {
-                const double d = A[sym_index(j, j)];
-                pd = pd && (d > 1e-13 * scale);
-                const double inv = recip(d);
-#pragma unroll
-                for (int i = j + 1; i < 9; ++i) {
-                    const double lij = A[sym_index(j, i)] * inv;
-#pragma unroll
-                    for (int c = i; c < 9; ++c)
-                        A[sym_index(i, c)] = fma(-lij, A[sym_index(j, c)], A[sym_index(i, c)]);
-                }
-            }
-            indefinite = !pd;
-        }
+        // Elements whose tangent is not certified positive definite (indefinite, or too close to singular to tell)
+        // get the eigen-clamp; for a positive semi-definite M it returns M itself.
+        if (kin_ok(k) && theta_ok(t.lam, t.mu))
+            indefinite = !eig9::tangent_certified_pd(k.F, k.J, t.lam, t.mu, fma(t.lam, k.L, -t.mu));
     }
     // One global atomic per workgroup: the wave counts are combined in LDS, and the indefinite lanes take
     // consecutive slots in (wave, lane) order after the workgroup's base.  A single counter hit once per wave

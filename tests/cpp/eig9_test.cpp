@@ -182,6 +182,71 @@ std::vector<double> neo_hookean_m(const double F[9], const double Dm[9], double 
     return M;
 }
 
+// Neo-Hookean tangent A(i+3j, m+3l) = mu d_im d_jl + lambda a_ij a_ml - p a_il a_mj (a = F^-T) as a full 9x9, and
+// the certificate's verdict on it against its eigenvalues (cyclic Jacobi).  A certified tangent must have no
+// eigenvalue below -1e-12 max|A|; an uncertified one should be indefinite or within 1e-6 max|A| of singular
+// (counted, not failed: the certificate may decline borderline cases).
+int g_cert_cases = 0, g_cert_pd = 0, g_cert_loose = 0;
+double g_cert_worst = 0;
+
+bool check_certificate(const double F[9], double lam, double mu, const char* what) {
+    double C[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            C[3 * i + j] = F[3 * ((i + 1) % 3) + (j + 1) % 3] * F[3 * ((i + 2) % 3) + (j + 2) % 3] -
+                           F[3 * ((i + 1) % 3) + (j + 2) % 3] * F[3 * ((i + 2) % 3) + (j + 1) % 3];
+    const double J = F[0] * C[0] + F[1] * C[1] + F[2] * C[2];
+    if (!(J > 0))
+        return true;
+    double a[9]; // F^-T = cof(F) / J
+    for (int q = 0; q < 9; ++q)
+        a[q] = C[q] / J;
+    const double p = lam * std::log(J) - mu;
+    std::vector<double> A(81);
+    double scale = 0;
+    for (int r = 0; r < 9; ++r)
+        for (int c = 0; c < 9; ++c) {
+            const int i = r % 3, j = r / 3, m = c % 3, l = c / 3;
+            double v = lam * a[3 * i + j] * a[3 * m + l] - p * a[3 * i + l] * a[3 * m + j];
+            if (i == m && j == l)
+                v += mu;
+            A[r * 9 + c] = v;
+            scale = std::max(scale, std::fabs(v));
+        }
+    std::vector<double> ev;
+    jacobi_eigenvalues(A, ev);
+    const bool cert = lpg::eig9::tangent_certified_pd(F, J, lam, mu, p);
+    ++g_cert_cases;
+    g_cert_pd += cert;
+    const double rel = ev[0] / scale;
+    if (cert && rel < -1e-12) {
+        std::printf("FAIL %s: certified positive definite, smallest eigenvalue %.3g max|A|\n", what, rel);
+        return false;
+    }
+    if (!cert && rel > 1e-6) {
+        ++g_cert_loose;
+        g_cert_worst = std::max(g_cert_worst, rel);
+    }
+    return true;
+}
+
+// A random rotation (Gram-Schmidt on a random 3x3), row-major.
+void random_rotation(std::mt19937_64& rng, double R[9]) {
+    std::uniform_real_distribution<double> U(-1, 1);
+    for (int q = 0; q < 9; ++q)
+        R[q] = U(rng);
+    for (int j = 0; j < 3; ++j) {
+        for (int i = 0; i < j; ++i) {
+            const double s = R[i] * R[j] + R[3 + i] * R[3 + j] + R[6 + i] * R[6 + j];
+            for (int k = 0; k < 3; ++k)
+                R[3 * k + j] -= s * R[3 * k + i];
+        }
+        const double n = std::sqrt(R[j] * R[j] + R[3 + j] * R[3 + j] + R[6 + j] * R[6 + j]);
+        for (int k = 0; k < 3; ++k)
+            R[3 * k + j] /= n;
+    }
+}
+
 } // namespace
 
 int main() {
@@ -266,6 +331,60 @@ int main() {
         const double lam = E * nu / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
         fails += !check(neo_hookean_m(F, Dm, lam, mu), "neo-Hookean");
         ++cases;
+    }
+    // Definiteness certificate of the tangent: random deformations of several amplitudes, stretched states like the
+    // 64^3 workload (20% x-stretch, nu = 0.45), and states placed on the pair boundary mu sigma_i sigma_j = |p|
+    // (F = U diag(sigma) V^T, mu solved for, then perturbed by a relative 1e-12 .. 1e-3 either way).
+    for (int t = 0; t < 20000; ++t) {
+        double F[9];
+        const double amp = (t % 4 == 0) ? 0.05 : (t % 4 == 1) ? 0.3 : (t % 4 == 2) ? 0.6 : 0.04;
+        for (int q = 0; q < 9; ++q)
+            F[q] = (q % 4 == 0 ? 1.0 : 0.0) + amp * U(rng);
+        if (t % 4 == 3)
+            F[0] += 0.2;
+        const double E = 1e5, nu = (t % 4 == 3) ? 0.45 : 0.3 + 0.19 * (0.5 + 0.5 * U(rng));
+        const double lam = E * nu / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
+        fails += !check_certificate(F, lam, mu, "certificate (random)");
+    }
+    for (int t = 0; t < 20000; ++t) {
+        double Ur[9], Vr[9], s[3], F[9];
+        random_rotation(rng, Ur);
+        random_rotation(rng, Vr);
+        for (double& v : s)
+            v = 0.5 + 1.5 * (0.5 + 0.5 * U(rng));
+        if (t % 2 == 0) // orientation-preserving rotations only
+            for (int k = 0; k < 3; ++k)
+                Ur[3 * k] = -Ur[3 * k];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                F[3 * i + j] = Ur[3 * i] * s[0] * Vr[3 * j] + Ur[3 * i + 1] * s[1] * Vr[3 * j + 1] +
+                               Ur[3 * i + 2] * s[2] * Vr[3 * j + 2];
+        double C[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                C[3 * i + j] = F[3 * ((i + 1) % 3) + (j + 1) % 3] * F[3 * ((i + 2) % 3) + (j + 2) % 3] -
+                               F[3 * ((i + 1) % 3) + (j + 2) % 3] * F[3 * ((i + 2) % 3) + (j + 1) % 3];
+        const double J = F[0] * C[0] + F[1] * C[1] + F[2] * C[2];
+        if (J <= 0)
+            for (int q = 0; q < 9; ++q)
+                F[q] = -F[q];
+        const double Jp = std::fabs(J), L = std::log(Jp);
+        const double smax = std::max(s[0], std::max(s[1], s[2]));
+        const double pair = Jp / smax; // the smallest sigma_i sigma_j
+        const double lam = 1e3 + 1e6 * (0.5 + 0.5 * U(rng));
+        // mu pair = |lam L - mu|: mu = lam L / (1 + pair) (p > 0) or mu = -lam L / (pair - 1) (p < 0, pair > 1).
+        double mu0 = L > 0 ? lam * L / (1 + pair) : (pair > 1 ? -lam * L / (pair - 1) : -1);
+        if (!(mu0 > 0))
+            mu0 = 1e4 * (1 + U(rng));
+        const double eps = std::pow(10.0, -12 + 9 * ((t / 2) % 4) / 3.0) * ((t / 8) % 2 ? 1 : -1);
+        fails += !check_certificate(F, lam, mu0 * (1 + eps), "certificate (pair boundary)");
+    }
+    std::printf("certificate: %d tangents, %d certified positive definite, %d uncertified although smallest "
+                "eigenvalue > 1e-6 max|A| (worst %.3g)\n",
+                g_cert_cases, g_cert_pd, g_cert_loose, g_cert_worst);
+    if (g_cert_loose * 200 > g_cert_cases) {
+        std::printf("FAIL certificate declines too many definite tangents\n");
+        ++fails;
     }
     std::printf("%d cases, %d failures; worst orthogonality %.3g, reconstruction %.3g, eigenvalues %.3g, "
                 "lean projection %.3g (relative to max |M|)\n",
