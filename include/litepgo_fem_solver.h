@@ -16,8 +16,9 @@ extern "C" {
 #endif
 
 /* NewtonOptions (solver/newton.hpp:13-20) plus the inner solver's controls: the reference factors the free
- * Hessian block with a direct solver; here each Newton system is solved by Jacobi-preconditioned CG to a relative
- * residual that tightens with the gradient (min(1e-2, |g| / |g_0|)) down to pcg_rtol. */
+ * Hessian block with a direct solver; here each Newton system is solved by CG preconditioned with the inverses of
+ * the 3x3 vertex diagonal blocks, to a relative residual that tightens with the gradient (min(1e-2, |g| / |g_0|))
+ * down to pcg_rtol. */
 typedef struct lpg_newton_options {
     double gradient_tolerance;     /* required: converged when the free gradient's max-norm is at most this */
     int32_t max_iterations;        /* 50 */

@@ -1,5 +1,5 @@
 // Equilibrium on the device: Newton's method with line search over the free DOFs (newton.cpp:112-183), the inner
-// linear systems solved by Jacobi-preconditioned CG on the device-assembled CSR values, and the equilibrium
+// linear systems solved by block-Jacobi-preconditioned CG on the device-assembled CSR values, and the equilibrium
 // sensitivities of static_equilibrium_response.cpp:43-88 (adjoint and tangent solves with the Exact Hessian).
 // The C ABI is include/litepgo_fem_solver.h.
 #include "litepgo_fem_solver.h"
@@ -11,6 +11,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <utility>
 
 using lpg::fail;
 using lpg::guarded;
@@ -35,19 +36,35 @@ void check_options(const lpg_newton_options& o) {
         fail(kInvalid, "newton_solve: pcg_rtol must lie in (0, 1) and pcg_max_iterations be positive");
 }
 
+// A device buffer freed when it goes out of scope (the solver's work space and temporaries).
+struct ScopedBuffer : DeviceBuffer {
+    ScopedBuffer() = default;
+    ScopedBuffer(const ScopedBuffer&) = delete;
+    ScopedBuffer& operator=(const ScopedBuffer&) = delete;
+    ~ScopedBuffer() {
+        release();
+    }
+    void swap(ScopedBuffer& o) {
+        std::swap(ptr, o.ptr);
+        std::swap(bytes, o.bytes);
+    }
+};
+
 // Device work space of one solve (released when the solve returns).
 struct Work {
     int n = 0;
-    DeviceBuffer g, H, d, r, z, p, q, minv, trial, part, scalars;
+    ScopedBuffer g, H, d, r, z, p, p_old, q, binv, trial, part, pcg_part, scalars;
     void* host = nullptr; // pinned read-back of 8 doubles
     explicit Work(const lpg_tet_energy* h) {
         const auto& s = h->setup;
         n = s.num_dofs;
         const size_t nd = std::max<size_t>(1, n) * sizeof(double);
-        for (DeviceBuffer* b : {&g, &d, &r, &z, &p, &q, &minv, &trial})
+        for (ScopedBuffer* b : {&g, &d, &r, &z, &p, &p_old, &q, &trial})
             b->ensure(nd);
+        binv.ensure(2 * nd); // 6 doubles per vertex
         H.ensure(std::max<size_t>(1, s.nnz) * sizeof(double));
         part.ensure(static_cast<size_t>(dot_partials()) * sizeof(double));
+        pcg_part.ensure(static_cast<size_t>(pcg_partials()) * sizeof(double));
         scalars.ensure(16 * sizeof(double));
         hip_check(hipHostMalloc(&host, 8 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
     }
@@ -68,8 +85,8 @@ struct Work {
     }
 };
 
-// Scalar slots: 0..4 PCG (r.z, p.q, new r.z, r.r, breakdown), 5 shift, 6 dot result, 7 second dot, 8 max-norm.
-constexpr int kShift = 5, kDot = 6, kMax = 8;
+// Scalar slots: 0..4 PCG (r.z ring of two, p.q, r.r, breakdown), 5 shift, 6 dot result, 7 second dot, 8 max-norm.
+constexpr int kRr = 3, kShift = 5, kDot = 6, kMax = 8;
 
 double device_dot(Work& w, const double* a, const double* b, hipStream_t s) {
     launch_dot(a, b, nullptr, nullptr, w.n, w.part.as<double>(), w.sc() + kDot, s);
@@ -87,41 +104,34 @@ struct PcgOutcome {
     double residual = 0.0; // |r|_2 at the end
 };
 
-// Solves (H_ff + shift I) d = -g_f by Jacobi-preconditioned CG from d = 0 to |r|_2 <= rtol |g_f|_2 (checked every
-// kCheckEvery iterations).  H and g are the handle's CSR values and a gradient (fixed entries ignored).
+// Solves (H_ff + shift I) d = -g_f by 3x3 block-Jacobi-preconditioned CG from d = 0 to |r|_2 <= rtol |g_f|_2
+// (checked every kCheckEvery iterations).  H and g are the handle's CSR values and a gradient (fixed entries ignored).
 PcgOutcome pcg(lpg_tet_energy* h, Work& w, const uint8_t* fixed, const double* g, double shift, double rtol,
                int max_iterations, hipStream_t s) {
     const DeviceMesh m = h->mesh();
-    const int n = w.n;
     double* sc = w.sc();
+    double* part = w.pcg_part.as<double>();
     hip_check(hipMemcpyAsync(sc + kShift, &shift, sizeof(double), hipMemcpyHostToDevice, s), "shift");
-    launch_jacobi(m, w.H.as<double>(), fixed, sc + kShift, w.minv.as<double>(), s);
-    launch_pcg_start(g, fixed, w.minv.as<double>(), w.r.as<double>(), w.d.as<double>(), w.z.as<double>(),
-                     w.p.as<double>(), n, s);
-    // s[0] = r.z, s[3] = r.r; s[4] = 0.
-    launch_dot(w.r.as<double>(), w.z.as<double>(), w.r.as<double>(), w.r.as<double>(), n, w.part.as<double>(), sc + 6,
-               s);
-    hip_check(hipMemcpyAsync(sc + 0, sc + 6, sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
-    hip_check(hipMemcpyAsync(sc + 3, sc + 7, sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
-    const double zero = 0.0;
-    hip_check(hipMemcpyAsync(sc + 4, &zero, sizeof(double), hipMemcpyHostToDevice, s), "flag");
-    const double rr0 = w.read(3, 1, s)[0];
+    launch_block_jacobi(m, w.H.as<double>(), fixed, sc + kShift, w.binv.as<double>(), s);
+    launch_pcg_begin(m, g, fixed, w.binv.as<double>(), w.r.as<double>(), w.d.as<double>(), w.z.as<double>(), sc, part,
+                     s);
+    launch_pcg_finish(m, sc, part, s);
+    const double rr0 = w.read(kRr, 1, s)[0];
     PcgOutcome out;
     const double target = rtol * rtol * rr0;
     out.residual = std::sqrt(rr0);
     if (!(rr0 > 0.0))
         return out; // g_f = 0: d = 0
     for (int it = 1; it <= max_iterations; ++it) {
-        launch_spmv(m, w.H.as<double>(), fixed, w.p.as<double>(), sc + kShift, w.q.as<double>(), s);
-        launch_dot(w.p.as<double>(), w.q.as<double>(), nullptr, nullptr, n, w.part.as<double>(), sc + 1, s);
-        launch_pcg_step(w.q.as<double>(), w.r.as<double>(), w.d.as<double>(), w.z.as<double>(), w.p.as<double>(),
-                        w.minv.as<double>(), sc, n, s);
-        launch_dot(w.r.as<double>(), w.z.as<double>(), w.r.as<double>(), w.r.as<double>(), n, w.part.as<double>(),
-                   sc + 2, s); // s[2] = new r.z, s[3] = r.r
-        launch_pcg_direction(w.z.as<double>(), w.p.as<double>(), sc, n, s);
+        w.p.swap(w.p_old);
+        launch_pcg_apply(m, w.H.as<double>(), fixed, w.z.as<double>(), w.p_old.as<double>(), w.p.as<double>(),
+                         w.q.as<double>(), sc + kShift, it, sc, part, s);
+        launch_pcg_update(m, w.p.as<double>(), w.q.as<double>(), w.d.as<double>(), w.r.as<double>(), w.z.as<double>(),
+                          w.binv.as<double>(), it, sc, part, s);
         out.iterations = it;
         if (it % kCheckEvery == 0 || it == max_iterations) {
-            const double* v = w.read(3, 2, s);
+            launch_pcg_finish(m, sc, part, s);
+            const double* v = w.read(kRr, 2, s); // r.r, breakdown flag
             out.residual = std::sqrt(std::max(0.0, v[0]));
             if (v[1] != 0.0) {
                 out.breakdown = true;
@@ -219,7 +229,7 @@ lpg_status lpg_tet_newton_solve_device(lpg_tet_energy* h, const uint8_t* fixed_d
             return;
         }
         double* x = x_dev;
-        lpg::DeviceBuffer gf; // reduced gradient F (g + b)
+        lpg::ScopedBuffer gf; // reduced gradient F (g + b)
         gf.ensure(static_cast<size_t>(n) * sizeof(double));
         double g0 = 0.0;
         for (int iteration = 0;; ++iteration) {
@@ -275,7 +285,7 @@ lpg_status lpg_tet_newton_solve_device(lpg_tet_energy* h, const uint8_t* fixed_d
             // Line search from the feasible step, shrinking until the energy decreases or stays within noise of it
             // (newton.cpp:87-112).
             double* dir = w.d.as<double>();
-            lpg::DeviceBuffer xt;
+            lpg::ScopedBuffer xt;
             xt.ensure(static_cast<size_t>(n) * sizeof(double));
             double alpha = lpg::feasible_step_call(h, x, dir, false, nullptr, s);
             bool accepted = false;
@@ -314,7 +324,7 @@ lpg_status lpg_tet_equilibrium_vjp_device(lpg_tet_energy* h, const uint8_t* fixe
         lpg::ensure_device(h);
         const auto s = static_cast<hipStream_t>(stream);
         lpg::Work w(h);
-        lpg::DeviceBuffer lam;
+        lpg::ScopedBuffer lam;
         lam.ensure(std::max<size_t>(1, w.n) * sizeof(double));
         if (w.n > 0)
             lpg::sensitivity_solve(h, w, fixed_dev, theta_dev, x_dev, w_dev, lam.as<double>(), *opt, s);
@@ -342,7 +352,7 @@ lpg_status lpg_tet_equilibrium_jvp_device(lpg_tet_energy* h, const uint8_t* fixe
         if (w.n == 0)
             return;
         // rhs = -(df/dtheta) dtheta (static_equilibrium_response.cpp:66-69).
-        lpg::DeviceBuffer rhs;
+        lpg::ScopedBuffer rhs;
         rhs.ensure(static_cast<size_t>(w.n) * sizeof(double));
         lpg::launch_clear(h->d_status(), nullptr, s);
         lpg::launch_force_jvp_theta(h->mesh(), x_dev, theta_dev, dtheta_dev, rhs.as<double>(), h->d_status(), s);
