@@ -68,18 +68,26 @@ def cpu_baseline(w, nthreads, repeats):
 
 
 def time_steps(dev, step, kind, steps, warmup):
+    """Wall time per step (the evaluation replayed as one captured hipGraph, no timing events), then the per-phase
+    kernel times from HIP events over a second run of the same steps (timing mode runs the launches eagerly)."""
     import torch
 
-    for _ in range(warmup):
+    dev.set_graph(True)
+    for _ in range(max(warmup, 2)):  # the second call captures the graph
         step(kind)
-    dev.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         step(kind)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    dev.set_graph(False)
+    dev.set_timing(True)
+    for _ in range(steps):
+        step(kind)
+    torch.cuda.synchronize()
     ms, evals = dev.kernel_times()
+    dev.set_timing(False)
     return elapsed / steps * 1e3, [m / evals for m in ms]
 
 
