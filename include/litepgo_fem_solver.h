@@ -17,8 +17,8 @@ extern "C" {
 
 /* NewtonOptions (solver/newton.hpp:13-20) plus the inner solver's controls: the reference factors the free
  * Hessian block with a direct solver; here each Newton system is solved by CG preconditioned with the inverses of
- * the 3x3 vertex diagonal blocks, to a relative residual that tightens with the gradient (min(1e-2, |g| / |g_0|))
- * down to pcg_rtol. */
+ * the 3x3 vertex diagonal blocks, to a relative residual given by the Eisenstat-Walker forcing term
+ * (0.9 (|g_k| / |g_k-1|)^2, safeguarded, at most 0.1), never below pcg_rtol. */
 typedef struct lpg_newton_options {
     double gradient_tolerance;     /* required: converged when the free gradient's max-norm is at most this */
     int32_t max_iterations;        /* 50 */

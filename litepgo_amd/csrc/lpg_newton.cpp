@@ -22,6 +22,7 @@ namespace {
 
 constexpr int kMaxDiagonalShifts = 10; // newton.cpp:15
 constexpr int kCheckEvery = 16;        // PCG iterations between convergence checks (one host read-back each)
+constexpr double kForcingMax = 0.1;    // largest relative residual asked of a Newton system's inner solve
 
 void check_options(const lpg_newton_options& o) {
     if (!(o.gradient_tolerance >= 0.0) || !std::isfinite(o.gradient_tolerance))
@@ -231,7 +232,7 @@ lpg_status lpg_tet_newton_solve_device(lpg_tet_energy* h, const uint8_t* fixed_d
         double* x = x_dev;
         lpg::ScopedBuffer gf; // reduced gradient F (g + b)
         gf.ensure(static_cast<size_t>(n) * sizeof(double));
-        double g0 = 0.0;
+        double eta = lpg::kForcingMax, gprev = 0.0;
         for (int iteration = 0;; ++iteration) {
             const double energy = lpg::total_energy(h, w, x, theta_dev, linear_dev, 1, o.hessian_kind, s);
             lpg::launch_free_combine(w.g.as<double>(), 1.0, linear_dev, 1.0, fixed_dev, gf.as<double>(), n, s);
@@ -249,11 +250,19 @@ lpg_status lpg_tet_newton_solve_device(lpg_tet_energy* h, const uint8_t* fixed_d
                 result->status = LPG_NEWTON_MAX_ITERATIONS;
                 return;
             }
-            if (iteration == 0)
-                g0 = gnorm;
-            // Inexact Newton: the inner tolerance tightens with the gradient (superlinear convergence) down to
-            // pcg_rtol.
-            const double rtol = std::max(o.pcg_rtol, std::min(1e-2, gnorm / g0));
+            // Inexact Newton with the Eisenstat-Walker forcing term (choice 2: eta = 0.9 (|g_k| / |g_k-1|)^2,
+            // safeguarded by 0.9 eta_k-1^2 when that exceeds 0.1, at most kForcingMax): loose inner solves while the
+            // line search limits progress, tightening as Newton converges, never below pcg_rtol.
+            if (iteration > 0) {
+                const double ratio = gnorm / gprev;
+                double next = 0.9 * ratio * ratio;
+                const double guard = 0.9 * eta * eta;
+                if (guard > 0.1)
+                    next = std::max(next, guard);
+                eta = std::min(lpg::kForcingMax, next);
+            }
+            gprev = gnorm;
+            const double rtol = std::max(o.pcg_rtol, eta);
             // Descent direction, shifting the diagonal while CG breaks down or the direction is not a descent one
             // (newton.cpp:57-77).
             bool found = false;
