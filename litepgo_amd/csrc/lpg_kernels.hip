@@ -941,6 +941,8 @@ __global__ __launch_bounds__(kPrepassThreads) void k_project_prepass(ProjectPara
 
 // The reduced element Hessian M = w (C (x) I3) A (C (x) I3)^T of tet e, packed upper: with c_m = sum_a P(a,m) g_a
 // and r_m = F^-T c_m, M[(3m+i), (3n+k)] = w [mu (c_m . c_n) d_ik + lambda r_m,i r_n,k - p r_n,i r_m,k].
+// ADD: M += the reduced Hessian (one rounding per entry) instead of M = it.
+template <bool ADD = false>
 __device__ __forceinline__ void reduced_hessian(const ProjectParams& p, int e, double (&M)[45]) {
     int ids[4];
     TetIn t;
@@ -971,7 +973,10 @@ __device__ __forceinline__ void reduced_hessian(const ProjectParams& p, int e, d
             double val = fma(t.lam * rm[m][i], rm[n][kk], -(pr * rm[n][i] * rm[m][kk]));
             if (i == kk)
                 val = fma(t.mu, dot3(cm[m][0], cm[m][1], cm[m][2], cm[n][0], cm[n][1], cm[n][2]), val);
-            M[sym_index(r, c)] = geo.w * val;
+            if (ADD)
+                M[sym_index(r, c)] = fma(geo.w, val, M[sym_index(r, c)]);
+            else
+                M[sym_index(r, c)] = geo.w * val;
         }
 }
 
@@ -990,10 +995,10 @@ __global__ __launch_bounds__(64, 2) void k_project_jacobi(ProjectParams p) {
     for (unsigned idx = blockIdx.x * 64 + threadIdx.x; idx < count; idx += gridDim.x * 64) {
         const int e = p.proj_list[idx];
         // The reduced Hessian is built twice (before the eigen-analysis and, when the projection adds to M, after
-        // it) rather than kept live through it, which keeps the eigen phase within the register budget.
-        const auto build_M = [&](double (&M)[45]) { reduced_hessian(p, e, M); };
+        // it, straight into the eigen-part C) rather than kept live through it, which keeps the eigen phase and
+        // the expansion below within the register budget.
         double M[45];
-        build_M(M);
+        reduced_hessian(p, e, M);
         double scale = 0.0;
 #pragma unroll
         for (int q = 0; q < 45; ++q)
@@ -1006,11 +1011,11 @@ __global__ __launch_bounds__(64, 2) void k_project_jacobi(ProjectParams p) {
         const bool converged = eig9::psd_part(
             M, Cp, add_M, scale, [](double x) { return rsqrt_nr(x); }, [](double x) { return recip(x); }, spill);
         if (add_M)
-            build_M(M);
+            reduced_hessian<true>(p, e, Cp);
         bool fin = true;
 #pragma unroll
         for (int q = 0; q < 45; ++q) {
-            M[q] = add_M ? M[q] + Cp[q] : Cp[q];
+            M[q] = Cp[q];
             fin = fin && isfinite(M[q]);
         }
         // proj(H)_ab = sum_mn P(a,m) P(b,n) proj(M)[(m,i),(n,k)].  For each slot a, store the 6 distinct entries of
