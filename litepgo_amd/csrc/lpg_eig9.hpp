@@ -212,9 +212,12 @@ constexpr double kStartVec[N][N] = {
     {1.18, 0.83, 0.56, 1.34, 1.01, 1.46, 0.67, 0.91, 1.26}, {0.96, 1.49, 1.14, 0.72, 0.85, 0.54, 1.23, 1.07, 0.79},
     {1.44, 0.65, 0.89, 1.08, 0.74, 1.37, 1.15, 0.51, 1.28}};
 
-// Number of values psd_part keeps in its spill store between the reduction and the back-transformation: the
-// Householder vectors (35) and their coefficients (7).
-constexpr int kSpill = (N - 1) * N / 2 - 1 + (N - 2);
+// Values psd_part keeps in its spill store: the Householder vectors (35) and their coefficients (7), needed again by
+// the back-transformation, and the tridiagonal matrix (9 + 8), needed again by the inverse iterations.
+constexpr int kSpillCoef = (N - 1) * N / 2 - 1;
+constexpr int kSpillD0 = kSpillCoef + (N - 2);
+constexpr int kSpillE0 = kSpillD0 + N;
+constexpr int kSpill = kSpillE0 + (N - 1);
 
 // a: packed upper triangle of the symmetric matrix M (destroyed).  On return proj(M) = (add_M ? M : 0) + C, with C
 // packed like a.  `spill` provides put(i, value) / get(i) for kSpill doubles (LDS on the device), which hold the
@@ -273,18 +276,18 @@ LPG_EIG_FN bool psd_part(double a[45], double C[45], bool& add_M, double scale, 
         for (int i = 0; i < N; ++i)
             if (i < m)
                 spill.put(off + i, v[i]);
-        spill.put(kSpill - (N - 2) + k, bk);
+        spill.put(kSpillCoef + k, bk);
     }
     e[N - 2] = a[sym(N - 2, N - 1)];
     e[N - 1] = 0.0;
 #pragma unroll
     for (int i = 0; i < N; ++i)
         d[i] = a[sym(i, i)];
-    double d0[N], e0[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        d0[i] = d[i];
-        e0[i] = e[i];
+        spill.put(kSpillD0 + i, d[i]);
+        if (i < N - 1)
+            spill.put(kSpillE0 + i, e[i]);
     }
 
     // ---- Eigenvalues: implicit QL on (d, e), as in sym_eig but without the basis.
@@ -381,10 +384,11 @@ LPG_EIG_FN bool psd_part(double a[45], double C[45], bool& add_M, double scale, 
         // pivot row, the multiplier and whether rows were swapped.
         double u1[N], u2[N], iu[N], mult[N - 1];
         bool swp[N - 1];
-        double c0 = d0[0] - lq, c1 = e0[0];
+        double c0 = spill.get(kSpillD0) - lq, c1 = spill.get(kSpillE0);
 #pragma unroll
         for (int i = 0; i < N - 1; ++i) {
-            const double n0 = e0[i], n1 = d0[i + 1] - lq, n2 = i + 2 < N ? e0[i + 1] : 0.0;
+            const double n0 = spill.get(kSpillE0 + i), n1 = spill.get(kSpillD0 + i + 1) - lq,
+                         n2 = i + 2 < N ? spill.get(kSpillE0 + i + 1) : 0.0;
             const bool sw = fabs(n0) > fabs(c0);
             double p0 = sw ? n0 : c0;
             const double p1 = sw ? n1 : c1, p2 = sw ? n2 : 0.0;
@@ -469,7 +473,7 @@ LPG_EIG_FN bool psd_part(double a[45], double C[45], bool& add_M, double scale, 
             for (int i = 0; i < N; ++i)
                 if (i < m)
                     t = fma(spill.get(off + i), x[k + 1 + i], t);
-            t *= spill.get(kSpill - (N - 2) + k);
+            t *= spill.get(kSpillCoef + k);
 #pragma unroll
             for (int i = 0; i < N; ++i)
                 if (i < m)

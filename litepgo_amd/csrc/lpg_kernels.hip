@@ -980,19 +980,23 @@ __device__ __forceinline__ void reduced_hessian(const ProjectParams& p, int e, d
         }
 }
 
-// Per-lane spill store of the eigen-clamp (the Householder reflectors, eig9::kSpill doubles), lane-interleaved in
-// LDS so that the 64 lanes of a wave touch 64 consecutive doubles per access.
+// Per-lane spill store of the eigen-clamp (eig9::kSpill doubles: the Householder reflectors and the tridiagonal
+// matrix), lane-interleaved in LDS so that the 64 lanes of a wave touch 64 consecutive doubles per access.
 struct LdsSpill {
     double* base; // &lds[lane]
     __device__ void put(int i, double v) { base[i * 64] = v; }
     __device__ double get(int i) const { return base[i * 64]; }
 };
 
-// One thread per indefinite element (grid-stride over the compacted list).
+// One thread per listed element.  A single pass (no grid-stride loop) keeps the compiler from hoisting the
+// element-independent constants of the eigen-analysis out of a loop and spilling them to scratch memory.
 __global__ __launch_bounds__(64, 2) void k_project_jacobi(ProjectParams p) {
     __shared__ double spill_lds[eig9::kSpill * 64];
     const unsigned count = *p.counter;
-    for (unsigned idx = blockIdx.x * 64 + threadIdx.x; idx < count; idx += gridDim.x * 64) {
+    {
+        const unsigned idx = blockIdx.x * 64 + threadIdx.x;
+        if (idx >= count)
+            return;
         const int e = p.proj_list[idx];
         // The reduced Hessian is built twice (before the eigen-analysis and, when the projection adds to M, after
         // it, straight into the eigen-part C) rather than kept live through it, which keeps the eigen phase and
@@ -1658,9 +1662,9 @@ void launch_project_jacobi(const DeviceMesh& m, const EvalArgs& a, hipStream_t s
         return;
     ProjectParams p{m.Xr, m.conn, m.theta, a.x, a.theta, m.num_tets, a.status, a.counter, a.proj_slot, a.proj_list, a.proj_H};
     p.fp = m.fp;
-    // Enough single-wave workgroups for two waves per SIMD on 256 CUs; the list length is only known on the device,
-    // so surplus workgroups exit at once and fewer elements than workgroups leave the rest idle.
-    const int blocks = std::min(8192, (m.num_tets + 63) / 64);
+    // One lane per element that could be listed: the list length is only known on the device, so the workgroups
+    // past it exit at once (the listed ones come first).
+    const int blocks = (m.num_tets + 63) / 64;
     hipLaunchKernelGGL(k_project_jacobi, dim3(blocks), dim3(64), 0, stream, p);
     check_launch("k_project_jacobi");
 }
