@@ -219,12 +219,15 @@ constexpr int kSpillD0 = kSpillCoef + (N - 2);
 constexpr int kSpillE0 = kSpillD0 + N;
 constexpr int kSpill = kSpillE0 + (N - 1);
 
-// a: packed upper triangle of the symmetric matrix M (destroyed).  On return proj(M) = (add_M ? M : 0) + C, with C
-// packed like a.  `spill` provides put(i, value) / get(i) for kSpill doubles (LDS on the device), which hold the
-// reflectors while the inverse iterations run.  Returns false if a QL sweep did not converge.
+// a: packed upper triangle of the symmetric matrix M (destroyed).  On return
+//   proj(M) = (add_M ? M : 0) + sum_{j < ns} wts[j] X[j] X[j]^T,
+// with ns <= 4 unit vectors X[j] and weights wts[j] >= 0 (|lambda| of the negative eigenvalues when add_M, else the
+// positive eigenvalues); X[j] and wts[j] for j >= ns are zero.  `spill` provides put(i, value) / get(i) for kSpill
+// doubles (LDS on the device), which hold the reflectors while the inverse iterations run.  Returns false if a QL
+// sweep did not converge.
 template <class Rsqrt, class Recip, class Spill>
-LPG_EIG_FN bool psd_part(double a[45], double C[45], bool& add_M, double scale, Rsqrt rsqrt, Recip recip,
-                         Spill& spill) {
+LPG_EIG_FN bool psd_part(double a[45], double X[4][N], double wts[4], int& ns, bool& add_M, double scale,
+                         Rsqrt rsqrt, Recip recip, Spill& spill) {
     double e[N], d[N];
     // ---- Householder reduction; reflector k (vector on indices k+1..N-1, coefficient beta_k) goes to the spill
     // store at offset off(k) = sum_{k' < k} (N - 1 - k'), the coefficients after the vectors.
@@ -358,7 +361,7 @@ LPG_EIG_FN bool psd_part(double a[45], double C[45], bool& add_M, double scale, 
     const bool use_neg = nneg <= 4;
     add_M = use_neg;
     double sl[4] = {0.0, 0.0, 0.0, 0.0};
-    int ns = 0;
+    ns = 0;
 #pragma unroll
     for (int q = 0; q < N; ++q) {
         const bool sel = use_neg ? d[q] < 0.0 : d[q] > 0.0;
@@ -452,10 +455,14 @@ LPG_EIG_FN bool psd_part(double a[45], double C[45], bool& add_M, double scale, 
             Y[j][i] = y[i];
     }
 
-    // ---- Back to the basis of M (x = H_0 H_1 ... H_{N-3} y) and C = sum_j |lambda_j| x_j x_j^T.
+    // ---- Back to the basis of M: x = H_0 H_1 ... H_{N-3} y.
 #pragma unroll
-    for (int i = 0; i < 45; ++i)
-        C[i] = 0.0;
+    for (int j = 0; j < 4; ++j) {
+        wts[j] = j < ns ? fabs(sl[j]) : 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            X[j][i] = 0.0;
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         if (!any_lane(j < ns))
@@ -479,14 +486,9 @@ LPG_EIG_FN bool psd_part(double a[45], double C[45], bool& add_M, double scale, 
                 if (i < m)
                     x[k + 1 + i] = fma(-t, spill.get(off + i), x[k + 1 + i]);
         }
-        const double w = j < ns ? fabs(sl[j]) : 0.0;
 #pragma unroll
-        for (int r = 0; r < N; ++r) {
-            const double wx = w * x[r];
-#pragma unroll
-            for (int c = r; c < N; ++c)
-                C[sym(r, c)] = fma(wx, x[c], C[sym(r, c)]);
-        }
+        for (int i = 0; i < N; ++i)
+            X[j][i] = x[i];
     }
     return ok;
 }

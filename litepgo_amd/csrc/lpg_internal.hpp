@@ -133,13 +133,15 @@ struct EvalArgs {
     unsigned* counter = nullptr; // indefinite count (device)
     int32_t* proj_slot = nullptr; // T
     int32_t* proj_list = nullptr; // T
-    double* proj_H = nullptr;    // T x kProjPerTet (slot-indexed, first *counter used)
+    double* proj_H = nullptr;    // T x kProjPerTet (list-indexed, first *counter used)
 };
 
-// Projected element blocks stored per indefinite tet: for each slot a, 6 distinct entries of the symmetric block
-// (a, a) and the three blocks (a, a^j), j = 1..3.
-constexpr int kProjPerSlot = 6 + 27;
-constexpr int kProjPerTet = 4 * kProjPerSlot;
+// Projection of a listed (not certified positive definite) tet as a low-rank term: proj(H_e) = (replace ? 0 : H_e)
+// + sum_{v < count} u_v u_v^T with at most kProjVectors 12-vectors u_v (element DOF order 3a + i).  Stored per list
+// entry as [count, 3 unused, u_0, u_1, u_2, u_3] (32-byte aligned); u_0 is written (zero) even when count is 0.
+// The tet's proj_slot holds its list index s (H_e plus the term) or -2 - s (the term replaces H_e); -1: not listed.
+constexpr int kProjVectors = 4;
+constexpr int kProjPerTet = 4 + 12 * kProjVectors;
 
 struct LaunchConfig {
     int lanes_per_vertex = 4; // lanes sharing one vertex's incidences

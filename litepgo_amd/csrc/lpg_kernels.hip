@@ -571,14 +571,16 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
                 gs1 += gc[1];
                 gs2 += gc[2];
                 es += ec;
-                // Indefinite tets (Projected) are added by the second pass below.
-                add = KIND == 0 || (KIND == 1 && slot < 0);
-                if (KIND == 1 && slot >= 0) {
+                // Projected: a listed tet's low-rank term is added by the second pass below; its exact blocks are
+                // added here unless the term replaces them (slot <= -2).
+                add = KIND == 0 || (KIND == 1 && slot >= -1);
+                if (KIND == 1 && slot != -1) {
+                    const int li = slot >= 0 ? slot : -2 - slot;
                     if (pend_s0 < 0) {
-                        pend_s0 = slot;
+                        pend_s0 = li;
                         pend_r0 = r;
                     } else if (pend_s1 < 0) {
-                        pend_s1 = slot;
+                        pend_s1 = li;
                         pend_r1 = r;
                     } else {
                         pend_all = true;
@@ -644,37 +646,68 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
             }
         }
         if (KIND == 1) {
-            // Second pass: indefinite tets add proj(H_e) as stored by the eigen-clamp kernel (for slot a: the 6
-            // distinct entries of block (a, a), then blocks (a, a^j), j = 1..3).  Keeping these loads out of the
-            // first pass keeps them from sharing registers with the exact values there.
+            // Second pass: listed tets add their low-rank term sum_v u_v u_v^T (stored by the eigen-clamp kernel):
+            // for slot a, block (a, a) (6 distinct entries) and blocks (a, a^j), j = 1..3.  Keeping these loads out
+            // of the first pass keeps them from sharing registers with the exact values there.
             const int steps = pend_all ? (ke - kb + LPV - 1) / LPV : 2;
             for (int q = 0; q < steps; ++q) {
                 bool add = false;
                 int cps[3] = {0, 0, 0};
-                const double* src = nullptr;
                 Inc r = q == 0 ? pend_r0 : pend_r1;
-                int slot = q == 0 ? pend_s0 : pend_s1;
+                int li = q == 0 ? pend_s0 : pend_s1;
                 if (pend_all) {
                     const int k = kb + sub + q * LPV;
-                    slot = -1;
+                    li = -1;
                     if (k < ke) {
                         r = inc[k];
-                        slot = p.proj_slot[r.tet];
+                        const int slot = p.proj_slot[r.tet];
+                        li = slot >= 0 ? slot : (slot == -1 ? -1 : -2 - slot);
                     }
                 }
-                double pv[kProjPerSlot];
-                if (slot >= 0) {
+                double pv[6 + 27];
+#pragma unroll
+                for (int c = 0; c < 6 + 27; ++c)
+                    pv[c] = 0.0;
+                if (li >= 0) {
                     add = true;
 #pragma unroll
                     for (int j = 1; j < 4; ++j)
                         cps[j - 1] = IncT<WIDE>::cp(r, j);
-                    src = p.proj_H + static_cast<size_t>(slot) * kProjPerTet + kProjPerSlot * IncT<WIDE>::slot(r);
-                    // All 33 loads are issued before any add (the empty asm orders them), so the entry costs one
-                    // memory latency rather than one per small group of loads.
+                    const int a = IncT<WIDE>::slot(r);
+                    const double* src = p.proj_H + static_cast<size_t>(li) * kProjPerTet;
+                    // u_0 is always present (zero when the count is 0), so its loads go out together with the count's;
+                    // further vectors (count > 1) are rare and loaded after it.
+                    double cnt = 0.0;
+                    for (int vv = 0; vv == 0 || vv < static_cast<int>(cnt); ++vv) {
+                        const double* u = src + 4 + 12 * vv;
+                        double ua[3], ub[3][3];
+                        // All loads of a vector (and, for u_0, the count) are issued before any product (the empty
+                        // asm orders them).
+                        if (vv == 0)
+                            cnt = src[0];
 #pragma unroll
-                    for (int c = 0; c < kProjPerSlot; ++c)
-                        pv[c] = src[c];
-                    asm volatile("" ::: "memory");
+                        for (int i = 0; i < 3; ++i)
+                            ua[i] = u[3 * a + i];
+#pragma unroll
+                        for (int j = 1; j < 4; ++j)
+#pragma unroll
+                            for (int k = 0; k < 3; ++k)
+                                ub[j - 1][k] = u[3 * (a ^ j) + k];
+                        asm volatile("" ::: "memory");
+                        pv[0] = fma(ua[0], ua[0], pv[0]);
+                        pv[1] = fma(ua[0], ua[1], pv[1]);
+                        pv[2] = fma(ua[0], ua[2], pv[2]);
+                        pv[3] = fma(ua[1], ua[1], pv[3]);
+                        pv[4] = fma(ua[1], ua[2], pv[4]);
+                        pv[5] = fma(ua[2], ua[2], pv[5]);
+#pragma unroll
+                        for (int j = 0; j < 3; ++j)
+#pragma unroll
+                            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                                for (int k = 0; k < 3; ++k)
+                                    pv[6 + 9 * j + 3 * i + k] = fma(ua[i], ub[j][k], pv[6 + 9 * j + 3 * i + k]);
+                    }
 #pragma unroll
                     for (int c = 0; c < 6; ++c)
                         ds[c] += pv[c];
@@ -998,78 +1031,47 @@ __global__ __launch_bounds__(64, 2) void k_project_jacobi(ProjectParams p) {
         if (idx >= count)
             return;
         const int e = p.proj_list[idx];
-        // The reduced Hessian is built twice (before the eigen-analysis and, when the projection adds to M, after
-        // it, straight into the eigen-part C) rather than kept live through it, which keeps the eigen phase and
-        // the expansion below within the register budget.
         double M[45];
         reduced_hessian(p, e, M);
         double scale = 0.0;
 #pragma unroll
         for (int q = 0; q < 45; ++q)
             scale = fmax(scale, fabs(M[q]));
-        // proj(M) = (add_M ? M : 0) + C from the eigenvalues and the smaller eigen-set's vectors (M is destroyed
-        // by the reduction and rebuilt when needed).
-        double Cp[45];
+        // proj(M) = (add_M ? M : 0) + sum_v wts_v x_v x_v^T from the eigenvalues and the smaller eigen-set's vectors
+        // (M is destroyed by the reduction).  With Q = P (x) I3, proj(H_e) = Q proj(M) Q^T = (add_M ? H_e : 0)
+        // + sum_v u_v u_v^T, u_v = sqrt(wts_v) Q x_v: the assembly adds H_e itself, so only the u_v are stored.
+        double X[4][9], wts[4];
+        int ns = 0;
         bool add_M = false;
         LdsSpill spill{spill_lds + threadIdx.x};
         const bool converged = eig9::psd_part(
-            M, Cp, add_M, scale, [](double x) { return rsqrt_nr(x); }, [](double x) { return recip(x); }, spill);
-        if (add_M)
-            reduced_hessian<true>(p, e, Cp);
+            M, X, wts, ns, add_M, scale, [](double x) { return rsqrt_nr(x); }, [](double x) { return recip(x); },
+            spill);
+        double* out = p.proj_H + kProjPerTet * static_cast<size_t>(idx);
+        out[0] = static_cast<double>(ns);
         bool fin = true;
 #pragma unroll
-        for (int q = 0; q < 45; ++q) {
-            M[q] = Cp[q];
-            fin = fin && isfinite(M[q]);
-        }
-        // proj(H)_ab = sum_mn P(a,m) P(b,n) proj(M)[(m,i),(n,k)].  For each slot a, store the 6 distinct entries of
-        // the symmetric block (a, a) and the blocks (a, a^j), j = 1..3 -- the values the incidence of the tet's
-        // vertex a adds to its rows.
-        double* out = p.proj_H + kProjPerTet * static_cast<size_t>(idx);
+        for (int v = 0; v < kProjVectors; ++v) {
+            if (!eig9::any_lane(v < ns || v == 0))
+                break;
+            const double sw = sqrt(wts[v]);
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            double Ta[3][9]; // Ta[n][3i+k] = sum_m P(a,m) proj(M)[(3m+i), (3n+k)]
+            for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int n = 0; n < 3; ++n)
+                for (int i = 0; i < 3; ++i) {
+                    double u = 0.0;
 #pragma unroll
-                for (int i = 0; i < 3; ++i)
-#pragma unroll
-                    for (int kk = 0; kk < 3; ++kk) {
-                        double s = 0.0;
-#pragma unroll
-                        for (int m = 0; m < 3; ++m)
-                            if (kP[a][m] != 0.0)
-                                s = fma(kP[a][m], M[sym_index(3 * m + i, 3 * n + kk)], s);
-                        Ta[n][3 * i + kk] = s;
-                    }
-            double* o = out + kProjPerSlot * a;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int b = a ^ j;
-                double blk[9];
-#pragma unroll
-                for (int c = 0; c < 9; ++c) {
-                    double s = 0.0;
-#pragma unroll
-                    for (int n = 0; n < 3; ++n)
-                        if (kP[b][n] != 0.0)
-                            s = fma(kP[b][n], Ta[n][c], s);
-                    blk[c] = s;
+                    for (int m = 0; m < 3; ++m)
+                        if (kP[a][m] != 0.0)
+                            u = fma(kP[a][m], X[v][3 * m + i], u);
+                    u *= sw;
+                    fin = fin && isfinite(u);
+                    if (v < ns || v == 0)
+                        out[4 + 12 * v + 3 * a + i] = u;
                 }
-                if (j == 0) {
-                    o[0] = blk[0];
-                    o[1] = blk[1];
-                    o[2] = blk[2];
-                    o[3] = blk[4];
-                    o[4] = blk[5];
-                    o[5] = blk[8];
-                } else {
-#pragma unroll
-                    for (int c = 0; c < 9; ++c)
-                        o[6 + 9 * (j - 1) + c] = blk[c];
-                }
-            }
         }
+        if (!add_M)
+            p.proj_slot[e] = -2 - static_cast<int32_t>(idx);
         if (!converged || !fin)
             atomicOr(p.status, kStatusProjection);
     }

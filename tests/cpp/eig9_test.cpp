@@ -78,7 +78,8 @@ bool check(const std::vector<double>& full, const char* what) {
         }
     // The lean PSD part (eigenvalues + inverse iteration for the smaller eigen-set) against V max(d, 0) V^T.
     {
-        double a2[45], C[45];
+        double a2[45], X[4][N], wts[4];
+        int ns = 0;
         for (int r = 0; r < N; ++r)
             for (int c = r; c < N; ++c)
                 a2[sym(r, c)] = full[r * N + c];
@@ -89,20 +90,25 @@ bool check(const std::vector<double>& full, const char* what) {
             double get(int i) const { return v[i]; }
         } spill;
         const bool ok2 = lpg::eig9::psd_part(
-            a2, C, add_M, scale, [](double x) { return 1.0 / std::sqrt(x); }, [](double x) { return 1.0 / x; },
-            spill);
+            a2, X, wts, ns, add_M, scale, [](double x) { return 1.0 / std::sqrt(x); },
+            [](double x) { return 1.0 / x; }, spill);
         double proj_err = 0;
         for (int r = 0; r < N; ++r)
             for (int c = r; c < N; ++c) {
                 double full_proj = 0;
                 for (int k = 0; k < N; ++k)
                     full_proj += V[r][k] * std::max(d[k], 0.0) * V[c][k];
-                const double lean = (add_M ? full[r * N + c] : 0.0) + C[sym(r, c)];
+                double lean = add_M ? full[r * N + c] : 0.0;
+                for (int j = 0; j < 4; ++j)
+                    lean += wts[j] * X[j][r] * X[j][c];
                 proj_err = std::max(proj_err, std::fabs(lean - full_proj));
             }
         proj_err /= scale;
         g_worst_proj = std::max(g_worst_proj, proj_err);
-        if (!ok2 || !(proj_err <= 1e-13)) {
+        bool tail_zero = ns >= 0 && ns <= 4;
+        for (int j = ns; j < 4 && tail_zero; ++j)
+            tail_zero = wts[j] == 0.0;
+        if (!ok2 || !(proj_err <= 1e-13) || !tail_zero) {
             std::printf("FAIL %s: lean projection error %.3g (converged %d)\n", what, proj_err, ok2 ? 1 : 0);
             return false;
         }
