@@ -142,6 +142,22 @@ def test_run_to_run_determinism():
     assert a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
 
 
+def test_projection_far_from_rest(oracle_mod):
+    """States whose tangents are indefinite everywhere: a 0.6x compression (3 negative eigenvalues per tet: the
+    projection adds a rank-3 term to the exact blocks) and 1.6x / 2x dilations (5 negative eigenvalues: the rank-4
+    positive part replaces the exact blocks).  Every tet is listed, so every lane takes the rescan path too."""
+    rng = np.random.default_rng(17)
+    X, tets = kuhn_grid(4)
+    lam, mu = lame_parameters(1e5, 0.45)
+    theta = np.tile([lam, mu], (len(tets), 1))
+    dev, ref = make(oracle_mod, X, tets, theta)
+    for scale in (-0.4, 0.6, 1.0):
+        print(f"uniform scale {1 + scale}", flush=True)
+        u = (scale * X + 0.0025 * rng.uniform(-1, 1, X.shape)).reshape(-1)
+        check_against_oracle(dev, ref, u)
+        assert dev.last_indefinite_count == len(tets)
+
+
 def test_irregular_numbering_and_unused_vertex(oracle_mod):
     rng = np.random.default_rng(21)
     X, tets = kuhn_grid((3, 4, 5))
