@@ -53,6 +53,15 @@ def test_equilibrium_matches_oracle(beam_case):
     assert abs(res.energy - e_ref) <= 1e-12 * abs(e_ref)
 
 
+def test_solve_is_reproducible_bitwise(torch, beam_case):
+    """Every reduction of the device solver has a fixed order: a second solve repeats the first bit for bit."""
+    b, dev, _, res, _, lin = beam_case
+    x0 = torch.zeros(3 * b.num_vertices, dtype=torch.float64, device="cuda")
+    again = solver.newton_solve(dev, b.fixed, x0, opts(), linear=lin)
+    assert again.iterations == res.iterations and again.pcg_iterations == res.pcg_iterations
+    assert torch.equal(again.x, res.x) and again.energy == res.energy
+
+
 def test_adjoint_and_tangent_match_oracle(torch, beam_case):
     from oracle import newton
 
