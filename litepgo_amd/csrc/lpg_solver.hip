@@ -105,9 +105,11 @@ __global__ __launch_bounds__(256) void k_dot_final(const double* __restrict__ pa
     }
 }
 
-// Maximum |a_i| over free DOFs (non-negative doubles order as integers), one atomic per wave.
+// Maximum |a_i| over free DOFs (non-negative doubles order as integers), one atomic per workgroup: atomics on a
+// single address serialize, so the waves' maxima are combined in LDS first.
 __global__ __launch_bounds__(256) void k_free_maxabs(const double* __restrict__ a, const uint8_t* __restrict__ fixed,
                                                      int n, unsigned long long* out_bits) {
+    __shared__ double wave_max[256 / kWave];
     double m = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += 256 * gridDim.x)
         if (!fixed[i])
@@ -116,7 +118,14 @@ __global__ __launch_bounds__(256) void k_free_maxabs(const double* __restrict__ 
     for (int off = kWave / 2; off > 0; off >>= 1)
         m = fmax(m, __shfl_xor(m, off));
     if ((threadIdx.x % kWave) == 0)
+        wave_max[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int w = 1; w < 256 / kWave; ++w)
+            m = fmax(m, wave_max[w]);
         atomicMax(out_bits, static_cast<unsigned long long>(__double_as_longlong(m)));
+    }
 }
 
 __global__ void k_set_bits(unsigned long long* p, double v) {
@@ -485,7 +494,7 @@ void launch_free_maxabs(const double* a, const uint8_t* fixed, int n, double* ou
     check_solver_launch("k_set_bits");
     if (n == 0)
         return;
-    hipLaunchKernelGGL(k_free_maxabs, dim3(std::min(1024u, blocks_for(n))), dim3(256), 0, s, a, fixed, n, bits);
+    hipLaunchKernelGGL(k_free_maxabs, dim3(std::min(256u, blocks_for(n))), dim3(256), 0, s, a, fixed, n, bits);
     check_solver_launch("k_free_maxabs");
 }
 
