@@ -212,12 +212,9 @@ constexpr double kStartVec[N][N] = {
     {1.18, 0.83, 0.56, 1.34, 1.01, 1.46, 0.67, 0.91, 1.26}, {0.96, 1.49, 1.14, 0.72, 0.85, 0.54, 1.23, 1.07, 0.79},
     {1.44, 0.65, 0.89, 1.08, 0.74, 1.37, 1.15, 0.51, 1.28}};
 
-// Values psd_part keeps in its spill store: the Householder vectors (35) and their coefficients (7), needed again by
-// the back-transformation, and the tridiagonal matrix (9 + 8), needed again by the inverse iterations.
-constexpr int kSpillCoef = (N - 1) * N / 2 - 1;
-constexpr int kSpillD0 = kSpillCoef + (N - 2);
-constexpr int kSpillE0 = kSpillD0 + N;
-constexpr int kSpill = kSpillE0 + (N - 1);
+// Values psd_part keeps in its spill store for the back-transformation: the Householder vectors, normalised so that
+// each reflector is I - 2 v v^T (35 values; 17.5 KB of LDS per 64-lane wave).
+constexpr int kSpill = (N - 1) * N / 2 - 1;
 
 // a: packed upper triangle of the symmetric matrix M (destroyed).  On return
 //   proj(M) = (add_M ? M : 0) + sum_{j < ns} wts[j] X[j] X[j]^T,
@@ -229,8 +226,8 @@ template <class Rsqrt, class Recip, class Spill>
 LPG_EIG_FN bool psd_part(double a[45], double X[4][N], double wts[4], int& ns, bool& add_M, double scale,
                          Rsqrt rsqrt, Recip recip, Spill& spill) {
     double e[N], d[N];
-    // ---- Householder reduction; reflector k (vector on indices k+1..N-1, coefficient beta_k) goes to the spill
-    // store at offset off(k) = sum_{k' < k} (N - 1 - k'), the coefficients after the vectors.
+    // ---- Householder reduction; reflector k (unit vector on indices k+1..N-1) goes to the spill store at offset
+    // off(k) = sum_{k' < k} (N - 1 - k').
 #pragma unroll
     for (int k = 0; k < N - 2; ++k) {
         const int m = N - 1 - k;
@@ -275,22 +272,23 @@ LPG_EIG_FN bool psd_part(double a[45], double X[4][N], double wts[4], int& ns, b
                 if (i < m && j < m && j >= i)
                     a[sym(k + 1 + i, k + 1 + j)] =
                         fma(-v[i], pw[j], fma(-pw[i], v[j], a[sym(k + 1 + i, k + 1 + j)]));
+        // Stored as v / |v| (H_k = I - 2 v v^T); a zero vector when the step is the identity (sigma == 0).
+        const double vs = sigma > 0.0 ? rsqrt(vtv) : 0.0;
 #pragma unroll
         for (int i = 0; i < N; ++i)
             if (i < m)
-                spill.put(off + i, v[i]);
-        spill.put(kSpillCoef + k, bk);
+                spill.put(off + i, v[i] * vs);
     }
     e[N - 2] = a[sym(N - 2, N - 1)];
     e[N - 1] = 0.0;
 #pragma unroll
     for (int i = 0; i < N; ++i)
         d[i] = a[sym(i, i)];
+    double d0[N], e0[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        spill.put(kSpillD0 + i, d[i]);
-        if (i < N - 1)
-            spill.put(kSpillE0 + i, e[i]);
+        d0[i] = d[i];
+        e0[i] = e[i];
     }
 
     // ---- Eigenvalues: implicit QL on (d, e), as in sym_eig but without the basis.
@@ -387,11 +385,10 @@ LPG_EIG_FN bool psd_part(double a[45], double X[4][N], double wts[4], int& ns, b
         // pivot row, the multiplier and whether rows were swapped.
         double u1[N], u2[N], iu[N], mult[N - 1];
         bool swp[N - 1];
-        double c0 = spill.get(kSpillD0) - lq, c1 = spill.get(kSpillE0);
+        double c0 = d0[0] - lq, c1 = e0[0];
 #pragma unroll
         for (int i = 0; i < N - 1; ++i) {
-            const double n0 = spill.get(kSpillE0 + i), n1 = spill.get(kSpillD0 + i + 1) - lq,
-                         n2 = i + 2 < N ? spill.get(kSpillE0 + i + 1) : 0.0;
+            const double n0 = e0[i], n1 = d0[i + 1] - lq, n2 = i + 2 < N ? e0[i + 1] : 0.0;
             const bool sw = fabs(n0) > fabs(c0);
             double p0 = sw ? n0 : c0;
             const double p1 = sw ? n1 : c1, p2 = sw ? n2 : 0.0;
@@ -480,7 +477,7 @@ LPG_EIG_FN bool psd_part(double a[45], double X[4][N], double wts[4], int& ns, b
             for (int i = 0; i < N; ++i)
                 if (i < m)
                     t = fma(spill.get(off + i), x[k + 1 + i], t);
-            t *= spill.get(kSpillCoef + k);
+            t *= 2.0;
 #pragma unroll
             for (int i = 0; i < N; ++i)
                 if (i < m)
