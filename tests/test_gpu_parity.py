@@ -158,6 +158,28 @@ def test_projection_far_from_rest(oracle_mod):
         assert dev.last_indefinite_count == len(tets)
 
 
+def test_real_mesh_from_veg(oracle_mod):
+    """The reference's box-tet.veg asset (243 vertices, 827 irregular tets), read with litepgo_amd.read_veg: the
+    state of the reference's own test (u = 1e-3 sin(i)) and a 30-degree twist about the long axis."""
+    from litepgo_amd import read_veg
+
+    mesh = read_veg(GOLD / "box-tet.veg")
+    lam, mu = lame_parameters(1e5, 0.4)
+    theta = np.tile([lam, mu], (len(mesh.cells), 1))
+    dev, ref = make(oracle_mod, mesh.positions, mesh.cells, theta)
+    n = 3 * mesh.num_vertices
+    check_against_oracle(dev, ref, 1e-3 * np.sin(np.arange(n)))
+    X = mesh.positions
+    z = (X[:, 2] - X[:, 2].min()) / np.ptp(X[:, 2])
+    ang = np.deg2rad(30.0) * z
+    cx, cy = X[:, 0].mean(), X[:, 1].mean()
+    rx, ry = X[:, 0] - cx, X[:, 1] - cy
+    u = np.zeros_like(X)
+    u[:, 0] = np.cos(ang) * rx - np.sin(ang) * ry - rx
+    u[:, 1] = np.sin(ang) * rx + np.cos(ang) * ry - ry
+    check_against_oracle(dev, ref, u.reshape(-1))
+
+
 def test_irregular_numbering_and_unused_vertex(oracle_mod):
     rng = np.random.default_rng(21)
     X, tets = kuhn_grid((3, 4, 5))
