@@ -120,6 +120,22 @@ lpg_status lpg_tet_energy_force_jvp_theta_device(lpg_tet_energy* energy, const d
                                                  void* stream);
 lpg_status lpg_tet_energy_hvp_device(lpg_tet_energy* energy, const double* x_dev, const double* theta_dev,
                                      const double* v_dev, double* out_dev, void* stream);
+/* Force response to the plastic parts (VolumetricFemForceFpResponse, fem/assembly/volumetric_fem_force_fp_response.hpp:
+ * detail::VolumetricFemForceResponse with ForceInput::Plastic, fem/assembly/detail/volumetric_fem_force_response.hpp:
+ * 81-179, and NeoHookean::eval_d2W_dF_dFp, neo_hookean.cpp:103-128): f(Fp) = dE/dx at fixed x and theta, as a
+ * function of the per-tet plastic parts.  fp_dev: num_tets x 9, column-major vec(Fp) per tet (det Fp > 0), or NULL
+ * for the handle's own Fp.  theta_dev may be NULL (snapshot theta).
+ *   force_fp    : out = f (num_dofs);
+ *   force_vjp_fp: out_e = (d f / d vec(Fp_e))^T w (num_tets x 9, column-major);
+ *   force_jvp_fp: out = sum_e (d f / d vec(Fp_e)) dFp_e (num_dofs), dfp_dev num_tets x 9 column-major. */
+lpg_status lpg_tet_energy_force_fp_device(lpg_tet_energy* energy, const double* x_dev, const double* theta_dev,
+                                          const double* fp_dev, double* out_dev, void* stream);
+lpg_status lpg_tet_energy_force_vjp_fp_device(lpg_tet_energy* energy, const double* x_dev, const double* theta_dev,
+                                              const double* fp_dev, const double* w_dev, double* out_dev,
+                                              void* stream);
+lpg_status lpg_tet_energy_force_jvp_fp_device(lpg_tet_energy* energy, const double* x_dev, const double* theta_dev,
+                                              const double* fp_dev, const double* dfp_dev, double* out_dev,
+                                              void* stream);
 /* Reverse product of (E, f = dE/dx) in one call, the fused backward of litepgo.torch's energy and response
  * adapters (src/python/torch.py:139-142 gives upstream * gradient, :154-157 the response VJP):
  *   gx     = s f + H w                       (num_dofs; H the Exact Hessian),

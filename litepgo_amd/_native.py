@@ -29,6 +29,9 @@ SIGNATURES = {
     "lpg_tet_energy_dE_dtheta_device": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "lpg_tet_energy_force_vjp_theta_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "lpg_tet_energy_force_jvp_theta_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "lpg_tet_energy_force_fp_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "lpg_tet_energy_force_vjp_fp_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "lpg_tet_energy_force_jvp_fp_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "lpg_tet_energy_hvp_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "lpg_tet_energy_vjp_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "lpg_tet_energy_set_launch": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32]),

@@ -90,7 +90,7 @@ lpg_tet_energy::~lpg_tet_energy() {
     graph.reset();
     for (lpg::DeviceBuffer* b : {&d_Xr, &d_conn, &d_theta, &d_inc_ptr, &d_inc, &d_nbr_ptr, &d_nbr, &d_self_pos, &d_row_ptr, &d_scalars, &d_vertex_E, &d_chunk_begin,
                                  &d_chunk_sum, &d_seg_chunk, &d_proj_slot, &d_proj_list, &d_proj_H, &d_x, &d_g, &d_H,
-                                 &d_dx, &d_seg_tbeg, &d_seg_all, &d_seg_energy, &d_fp,
+                                 &d_dx, &d_seg_tbeg, &d_seg_all, &d_seg_energy, &d_fp, &d_fp_elem,
                                  &plan.d_tile_vbeg, &plan.d_normal_ids, &plan.d_oversized_ids})
         b->release();
     if (pinned)
@@ -570,6 +570,36 @@ lpg_status lpg_tet_energy_force_jvp_theta_device(lpg_tet_energy* h, const double
                                 [&](const lpg::DeviceMesh& m, unsigned* st, hipStream_t s) {
                                     lpg::launch_force_jvp_theta(m, x_dev, theta_dev, dtheta_dev, out_dev, st, s);
                                 });
+}
+
+lpg_status lpg_tet_energy_force_fp_device(lpg_tet_energy* h, const double* x_dev, const double* theta_dev,
+                                          const double* fp_dev, double* out_dev, void* stream) {
+    return lpg::derivative_call(h, x_dev, theta_dev, stream, [&](const lpg::DeviceMesh& m, unsigned* st, hipStream_t s) {
+        h->d_fp_elem.ensure(12 * static_cast<size_t>(m.num_tets) * sizeof(double));
+        lpg::launch_force_fp(m, x_dev, theta_dev, fp_dev, nullptr, nullptr, h->d_fp_elem.as<double>(), out_dev, st, s);
+    });
+}
+
+lpg_status lpg_tet_energy_force_vjp_fp_device(lpg_tet_energy* h, const double* x_dev, const double* theta_dev,
+                                              const double* fp_dev, const double* w_dev, double* out_dev,
+                                              void* stream) {
+    return lpg::derivative_call(h, x_dev, theta_dev, stream, [&](const lpg::DeviceMesh& m, unsigned* st, hipStream_t s) {
+        if (!w_dev && m.num_tets > 0)
+            lpg::fail(lpg::kInvalid, "force_vjp_fp: w must have num_dofs entries");
+        lpg::launch_force_fp(m, x_dev, theta_dev, fp_dev, w_dev, nullptr, nullptr, out_dev, st, s);
+    });
+}
+
+lpg_status lpg_tet_energy_force_jvp_fp_device(lpg_tet_energy* h, const double* x_dev, const double* theta_dev,
+                                              const double* fp_dev, const double* dfp_dev, double* out_dev,
+                                              void* stream) {
+    return lpg::derivative_call(h, x_dev, theta_dev, stream, [&](const lpg::DeviceMesh& m, unsigned* st, hipStream_t s) {
+        if (!dfp_dev && m.num_tets > 0)
+            lpg::fail(lpg::kInvalid, "force_jvp_fp: dFp must have num_tets x 9 entries");
+        h->d_fp_elem.ensure(12 * static_cast<size_t>(m.num_tets) * sizeof(double));
+        lpg::launch_force_fp(m, x_dev, theta_dev, fp_dev, nullptr, dfp_dev, h->d_fp_elem.as<double>(), out_dev, st,
+                             s);
+    });
 }
 
 lpg_status lpg_tet_energy_hvp_device(lpg_tet_energy* h, const double* x_dev, const double* theta_dev,

@@ -1156,6 +1156,302 @@ template <bool HAS_S, bool HAS_W> __global__ __launch_bounds__(256) void k_theta
         atomicOr(p.status, flags);
 }
 
+// Force response to the plastic part (VolumetricFemForceFpResponse = detail::VolumetricFemForceResponse with
+// ForceInput::Plastic, volumetric_fem_force_response.hpp:81-179; NeoHookean::eval_d2W_dF_dFp, neo_hookean.cpp:103-128):
+// f(Fp) = dE/dx at fixed x and theta as a function of the per-tet plastic parts (column-major vec(Fp), 9 per tet).
+// With Fe = F Fp^-1, psi_Fe = mu Fe + p Fe^-T (p = lambda log det Fe - mu), w' = w det Fp and g'_b = Fp^-T g_b (g_b
+// the rest shape gradients), the element force is f_b = w' psi_Fe g'_b, and for a direction dFp
+//   df_b = w' D g'_b,  D = tr(Fp^-1 dFp) psi_Fe + A:dFe - psi_Fe Fp^-T dFp^T,  dFe = -Fe dFp Fp^-1,
+// A:X = mu X + lambda (Fe^-T : X) Fe^-T - p Fe^-T X^T Fe^-T (the tangent of neo_hookean.cpp:43-60).  The transpose
+// product for a force cotangent w (V' = sum_b w_b g'_b^T, c = psi_Fe : V') is
+//   (df/dFp)^T w = w' [c I - Fe^T (A:V') - V'^T psi_Fe] Fp^-T.
+struct FpParams {
+    const double* Xr;
+    const int32_t* conn;
+    const double* theta_snap;
+    const double* x;
+    const double* theta;
+    const double* fp_snap; // the handle's Fp^-1 (row-major) and det Fp per tet, or null (identity)
+    const double* fp;      // per-call Fp (column-major vec, 9 per tet) or null
+    const double* w;       // VJP: force cotangent (3 per vertex)
+    const double* dfp;     // element mode: direction (9 per tet), or null for the force itself
+    double* out;           // VJP: 9 per tet; element mode: 12 per tet
+    unsigned* status;
+    int num_tets;
+};
+
+// Per-tet state of the Fp response: rest gradients g_b (rows of G), Fp^-1 (row-major), w' = w det Fp, Fe, Fe^-T,
+// p and psi_Fe (row-major).
+struct FpState {
+    double G[4][3];
+    double Fi[9];
+    double wp, p, mu, lam;
+    double Fe[9], a[9], psi[9];
+};
+
+__device__ __forceinline__ void fp_state(const FpParams& p, int e, int ids[4], FpState& s, unsigned& flags) {
+    TetIn t;
+    load_tet(p.Xr, p.conn, p.theta_snap, p.x, p.theta, e, ids, t, nullptr);
+    Geo geo;
+    rest_geometry(t, geo); // rest geometry without the plastic part: Di = Dm^-1, w = det Dm / 6
+    shape_gradients(geo.Di, s.G);
+    double detp = 1.0;
+    if (p.fp) {
+        double Fp[9]; // row-major from the column-major input
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                Fp[3 * i + j] = p.fp[9 * static_cast<size_t>(e) + i + 3 * j];
+        double C[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+                C[3 * i + j] = fma(Fp[3 * i1 + j1], Fp[3 * i2 + j2], -(Fp[3 * i1 + j2] * Fp[3 * i2 + j1]));
+            }
+        detp = dot3(Fp[0], Fp[1], Fp[2], C[0], C[1], C[2]);
+        if (!(detp > 0.0) || !isfinite(detp))
+            flags |= kStatusDetF; // MultiplicativeKinematics: det Fp must be positive and finite
+        const double inv = 1.0 / detp;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                s.Fi[3 * i + j] = C[3 * j + i] * inv;
+    } else if (p.fp_snap) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+            s.Fi[q] = p.fp_snap[10 * static_cast<size_t>(e) + q];
+        detp = p.fp_snap[10 * static_cast<size_t>(e) + 9];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+            s.Fi[q] = (q % 4 == 0) ? 1.0 : 0.0;
+    }
+    s.wp = geo.w * detp;
+    s.mu = t.mu;
+    s.lam = t.lam;
+    if (!theta_ok(t.lam, t.mu))
+        flags |= kStatusBadTheta;
+    // F = I + Du Dm^-1 (total), Fe = F Fp^-1.
+    double F[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double v = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                v = fma(t.u[c + 1][i] - t.u[0][i], geo.Di[3 * c + j], v);
+            F[3 * i + j] = v;
+        }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            s.Fe[3 * i + j] = dot3(F[3 * i], F[3 * i + 1], F[3 * i + 2], s.Fi[j], s.Fi[3 + j], s.Fi[6 + j]);
+    double C[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            C[3 * i + j] = fma(s.Fe[3 * i1 + j1], s.Fe[3 * i2 + j2], -(s.Fe[3 * i1 + j2] * s.Fe[3 * i2 + j1]));
+        }
+    const double J = dot3(s.Fe[0], s.Fe[1], s.Fe[2], C[0], C[1], C[2]);
+    const double L = log(J);
+    if (!(J > 0.0 && isfinite(J) && isfinite(L)))
+        flags |= kStatusDetF;
+    const double inv = 1.0 / J;
+    s.p = fma(t.lam, L, -t.mu);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        s.a[q] = C[q] * inv;
+        s.psi[q] = fma(t.mu, s.Fe[q], s.p * s.a[q]);
+    }
+}
+
+// A:X for the neo-Hookean tangent at Fe (row-major 3x3 arguments).
+__device__ __forceinline__ void tangent_apply(const FpState& s, const double X[9], double Y[9]) {
+    double ax = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+        ax = fma(s.a[q], X[q], ax);
+    double XTa[9]; // X^T a
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            XTa[3 * i + j] = dot3(X[i], X[3 + i], X[6 + i], s.a[j], s.a[3 + j], s.a[6 + j]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double aXTa = dot3(s.a[3 * i], s.a[3 * i + 1], s.a[3 * i + 2], XTa[j], XTa[3 + j], XTa[6 + j]);
+            Y[3 * i + j] = fma(s.mu, X[3 * i + j], fma(s.lam * ax, s.a[3 * i + j], -(s.p * aXTa)));
+        }
+}
+
+// g'_b = Fp^-T g_b.
+__device__ __forceinline__ void plastic_gradient(const FpState& s, int b, double gp[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        gp[i] = dot3(s.Fi[i], s.Fi[3 + i], s.Fi[6 + i], s.G[b][0], s.G[b][1], s.G[b][2]);
+}
+
+__global__ __launch_bounds__(256) void k_fp_vjp(FpParams p) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= p.num_tets)
+        return;
+    unsigned flags = 0;
+    int ids[4];
+    FpState s;
+    fp_state(p, e, ids, s, flags);
+    double V[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}; // sum_b w_b g'_b^T
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        double gp[3];
+        plastic_gradient(s, b, gp);
+        const double* wb = p.w + 3 * static_cast<size_t>(ids[b]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                V[3 * i + j] = fma(wb[i], gp[j], V[3 * i + j]);
+    }
+    double c = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+        c = fma(s.psi[q], V[q], c);
+    double Y[9];
+    tangent_apply(s, V, Y);
+    double Mt[9]; // c I - Fe^T Y - V^T psi
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double fy = dot3(s.Fe[i], s.Fe[3 + i], s.Fe[6 + i], Y[j], Y[3 + j], Y[6 + j]);
+            const double vp = dot3(V[i], V[3 + i], V[6 + i], s.psi[j], s.psi[3 + j], s.psi[6 + j]);
+            Mt[3 * i + j] = ((i == j) ? c : 0.0) - fy - vp;
+        }
+    double* out = p.out + 9 * static_cast<size_t>(e);
+    bool fin = true;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            // (M Fp^-T)(i, j) = sum_k M(i, k) Fp^-1(j, k); column-major output.
+            const double v = s.wp * dot3(Mt[3 * i], Mt[3 * i + 1], Mt[3 * i + 2], s.Fi[3 * j], s.Fi[3 * j + 1],
+                                         s.Fi[3 * j + 2]);
+            fin = fin && isfinite(v);
+            out[i + 3 * j] = v;
+        }
+    if (!fin)
+        flags |= kStatusNonFiniteOut;
+    if (flags)
+        atomicOr(p.status, flags);
+}
+
+// Element contributions (12 per tet, vertex slot-major): the force f_b itself (dfp null) or df_b for dfp.
+__global__ __launch_bounds__(256) void k_fp_elem(FpParams p) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= p.num_tets)
+        return;
+    unsigned flags = 0;
+    int ids[4];
+    FpState s;
+    fp_state(p, e, ids, s, flags);
+    double D[9];
+    if (!p.dfp) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+            D[q] = s.psi[q];
+    } else {
+        double dF[9]; // row-major from column-major
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                dF[3 * i + j] = p.dfp[9 * static_cast<size_t>(e) + i + 3 * j];
+        double tr = 0.0; // tr(Fp^-1 dFp)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            tr += dot3(s.Fi[3 * i], s.Fi[3 * i + 1], s.Fi[3 * i + 2], dF[i], dF[3 + i], dF[6 + i]);
+        double T1[9], dFe[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                T1[3 * i + j] = dot3(s.Fe[3 * i], s.Fe[3 * i + 1], s.Fe[3 * i + 2], dF[j], dF[3 + j], dF[6 + j]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                dFe[3 * i + j] = -dot3(T1[3 * i], T1[3 * i + 1], T1[3 * i + 2], s.Fi[j], s.Fi[3 + j], s.Fi[6 + j]);
+        double AdFe[9];
+        tangent_apply(s, dFe, AdFe);
+        double PF[9]; // psi Fp^-T
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                PF[3 * i + j] = dot3(s.psi[3 * i], s.psi[3 * i + 1], s.psi[3 * i + 2], s.Fi[3 * j], s.Fi[3 * j + 1],
+                                     s.Fi[3 * j + 2]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                // (psi Fp^-T dFp^T)(i, j) = sum_k PF(i, k) dFp(j, k)
+                const double pfd = dot3(PF[3 * i], PF[3 * i + 1], PF[3 * i + 2], dF[3 * j], dF[3 * j + 1], dF[3 * j + 2]);
+                D[3 * i + j] = fma(tr, s.psi[3 * i + j], AdFe[3 * i + j] - pfd);
+            }
+    }
+    double* out = p.out + 12 * static_cast<size_t>(e);
+    bool fin = true;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        double gp[3];
+        plastic_gradient(s, b, gp);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double v = s.wp * dot3(D[3 * i], D[3 * i + 1], D[3 * i + 2], gp[0], gp[1], gp[2]);
+            fin = fin && isfinite(v);
+            out[3 * b + i] = v;
+        }
+    }
+    if (!fin)
+        flags |= kStatusNonFiniteOut;
+    if (flags)
+        atomicOr(p.status, flags);
+}
+
+// out_v = sum over the incidences of v, in list order, of the element values of its slot (12 per tet).
+__global__ __launch_bounds__(256) void k_elem_gather(const int32_t* __restrict__ inc_ptr,
+                                                     const int32_t* __restrict__ inc_tet, int inc_stride,
+                                                     const int32_t* __restrict__ conn,
+                                                     const double* __restrict__ elem, double* __restrict__ out,
+                                                     int num_vertices) {
+    const int v = blockIdx.x * 256 + threadIdx.x;
+    if (v >= num_vertices)
+        return;
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    for (int k = inc_ptr[v]; k < inc_ptr[v + 1]; ++k) {
+        const int e = *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(inc_tet) +
+                                                         static_cast<size_t>(k) * inc_stride);
+        const int4 c = reinterpret_cast<const int4*>(conn)[e];
+        const int a = (c.x == v) ? 0 : (c.y == v) ? 1 : (c.z == v) ? 2 : 3;
+        const double* src = elem + 12 * static_cast<size_t>(e) + 3 * a;
+        y0 += src[0];
+        y1 += src[1];
+        y2 += src[2];
+    }
+    out[3 * static_cast<size_t>(v)] = y0;
+    out[3 * static_cast<size_t>(v) + 1] = y1;
+    out[3 * static_cast<size_t>(v) + 2] = y2;
+}
+
 // Vertex-gather products over each vertex's incidence list:
 //   MODE 0: (H v)_v                                (Exact Hessian-vector product)
 //   MODE 1: sum_e (d g_e / d theta_e) dtheta_e     (force JVP in theta; v = dtheta, T x 2)
@@ -1730,6 +2026,31 @@ static void launch_gather(int mode, const DeviceMesh& m, const double* x, const 
 void launch_force_jvp_theta(const DeviceMesh& m, const double* x, const double* theta, const double* dtheta,
                             double* out, unsigned* status, hipStream_t stream) {
     launch_gather(1, m, x, theta, dtheta, nullptr, out, status, stream);
+}
+
+void launch_force_fp(const DeviceMesh& m, const double* x, const double* theta, const double* fp, const double* w,
+                     const double* dfp, double* scratch, double* out, unsigned* status, hipStream_t stream) {
+    FpParams p{m.Xr, m.conn, m.theta, x, theta, m.fp, fp, w, dfp, w ? out : scratch, status, m.num_tets};
+    const dim3 grid((m.num_tets + 255) / 256), block(256);
+    if (w) {
+        if (m.num_tets > 0) {
+            hipLaunchKernelGGL(k_fp_vjp, grid, block, 0, stream, p);
+            check_launch("k_fp_vjp");
+        }
+        return;
+    }
+    if (m.num_tets > 0) {
+        hipLaunchKernelGGL(k_fp_elem, grid, block, 0, stream, p);
+        check_launch("k_fp_elem");
+    }
+    if (m.num_vertices == 0)
+        return;
+    // Vertices with no incidences (every vertex when there are no tets) get zeros.
+    hipLaunchKernelGGL(k_elem_gather, dim3((m.num_vertices + 255) / 256), dim3(256), 0, stream, m.inc_ptr,
+                       static_cast<const int32_t*>(m.inc),
+                       m.wide ? static_cast<int>(sizeof(IncWide)) : static_cast<int>(sizeof(IncNarrow)), m.conn,
+                       scratch, out, m.num_vertices);
+    check_launch("k_elem_gather");
 }
 
 void launch_hvp(const DeviceMesh& m, const double* x, const double* theta, const double* v, double* out,

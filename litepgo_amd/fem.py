@@ -353,6 +353,45 @@ class VolumetricFemEnergy:
                       "force_jvp_theta")
         return out
 
+    # Force response to the plastic parts (VolumetricFemForceFpResponse): Fp is num_tets x 9, column-major vec(Fp)
+    # per tet (None: the energy's own Fp); x and theta stay fixed.
+    def force_fp_device(self, x, Fp=None, theta=None):
+        import torch
+
+        self._check_tensor(x, self.num_dofs, "x")
+        if Fp is not None:
+            self._check_tensor(Fp, 9 * self.num_tets, "Fp")
+        out = torch.empty(self.num_dofs, dtype=torch.float64, device=x.device)
+        _native.check(_native.lib().lpg_tet_energy_force_fp_device(self._h, _tptr(x), _tptr(theta), _tptr(Fp),
+                                                                   _tptr(out), _stream()), "force_fp")
+        return out
+
+    def force_vjp_fp_device(self, x, w, Fp=None, theta=None):
+        import torch
+
+        self._check_tensor(x, self.num_dofs, "x")
+        self._check_tensor(w, self.num_dofs, "w")
+        if Fp is not None:
+            self._check_tensor(Fp, 9 * self.num_tets, "Fp")
+        out = torch.empty((self.num_tets, 9), dtype=torch.float64, device=x.device)
+        _native.check(_native.lib().lpg_tet_energy_force_vjp_fp_device(self._h, _tptr(x), _tptr(theta), _tptr(Fp),
+                                                                       _tptr(w), _tptr(out), _stream()),
+                      "force_vjp_fp")
+        return out
+
+    def force_jvp_fp_device(self, x, dFp, Fp=None, theta=None):
+        import torch
+
+        self._check_tensor(x, self.num_dofs, "x")
+        self._check_tensor(dFp, 9 * self.num_tets, "dFp")
+        if Fp is not None:
+            self._check_tensor(Fp, 9 * self.num_tets, "Fp")
+        out = torch.empty(self.num_dofs, dtype=torch.float64, device=x.device)
+        _native.check(_native.lib().lpg_tet_energy_force_jvp_fp_device(self._h, _tptr(x), _tptr(theta), _tptr(Fp),
+                                                                       _tptr(dFp), _tptr(out), _stream()),
+                      "force_jvp_fp")
+        return out
+
     def hvp_device(self, x, v, theta=None):
         import torch
 

@@ -215,3 +215,73 @@ class OracleEnergy:
         buf = ctypes.create_string_buffer(512)
         _check(lib().orc_force_theta_products(self._h, _ptr(x), None, _ptr(dt), None, _ptr(out), buf, 512), buf)
         return out
+
+
+def d2W_dF_dFp(F, Fp, theta):
+    """NeoHookean::eval_d2W_dF_dFp (neo_hookean.cpp:103-128): 9x9, column c = vec(dP) (column-major) for the unit
+    dFp with column-major index c, where P = dW/dF = det(Fp) psi_Fe(Fe) Fp^-T and Fe = F Fp^-1:
+    dP = tr(Fp^-1 dFp) P + det(Fp) (A_e : dFe) Fp^-T - P dFp^T Fp^-T, dFe = -Fe dFp Fp^-1, A_e the tangent of psi
+    at Fe (the plastic-free point evaluation at Fe)."""
+    F = np.asarray(F, dtype=np.float64).reshape(3, 3)
+    Fp = np.asarray(Fp, dtype=np.float64).reshape(3, 3)
+    inv_Fp = np.linalg.inv(Fp)
+    Fe = F @ inv_Fp
+    _, P, _, _ = neohookean(F, Fp, theta, A=False, mixed=False)
+    _, _, Ae, _ = neohookean(Fe, np.eye(3), theta, P=False, mixed=False)
+    Jp = np.linalg.det(Fp)
+    out = np.zeros((9, 9))
+    for c in range(9):
+        dFp = np.zeros(9)
+        dFp[c] = 1.0
+        dFp = dFp.reshape(3, 3, order="F")
+        dFe = -Fe @ dFp @ inv_Fp
+        dS = (Ae @ dFe.reshape(-1, order="F")).reshape(3, 3, order="F")
+        dP = np.trace(inv_Fp @ dFp) * P + Jp * dS @ inv_Fp.T - P @ dFp.T @ inv_Fp.T
+        out[:, c] = dP.reshape(-1, order="F")
+    return out
+
+
+def force_fp_products(tets, rest, theta, x, Fp, w=None, dFp=None):
+    """VolumetricFemForceFpResponse<NeoHookean, TetP1Basis, TetCentroidQuadrature> restated per tet
+    (fem/assembly/detail/volumetric_fem_force_response.hpp:155-179 with ForceInput::Plastic): the force f(Fp) = dE/dx
+    at fixed x and theta, f_b = w_e P g_b (w_e = det(Dm) / 6, g_b the rest shape gradients), and its Jacobian in the
+    per-tet column-major vec(Fp), J_e = w_e B^T d2W_dF_dFp with vec(dF) = B du.
+
+    tets (T, 4), rest (V, 3), theta (T, 2), x (3V,), Fp (T, 9) column-major.  Returns (f, J^T w (T, 9) or None,
+    J dFp (3V,) or None)."""
+    tets = np.asarray(tets).reshape(-1, 4)
+    rest = np.asarray(rest, dtype=np.float64).reshape(-1, 3)
+    u = np.asarray(x, dtype=np.float64).reshape(-1, 3)
+    Fp = np.asarray(Fp, dtype=np.float64).reshape(-1, 9)
+    f = np.zeros(u.size)
+    jt = None if w is None else np.zeros((len(tets), 9))
+    jv = None if dFp is None else np.zeros(u.size)
+    wv = None if w is None else np.asarray(w, dtype=np.float64).reshape(-1, 3)
+    dv = None if dFp is None else np.asarray(dFp, dtype=np.float64).reshape(-1, 9)
+    for e, ids in enumerate(tets):
+        X = rest[ids]
+        Dm = (X[1:] - X[0]).T
+        Di = np.linalg.inv(Dm)
+        G = np.vstack([-Di.sum(axis=0), Di])  # rows g_0..g_3
+        we = np.linalg.det(Dm) / 6.0
+        F = np.eye(3) + sum(np.outer(u[ids[b]], G[b]) for b in range(4))
+        Fpe = Fp[e].reshape(3, 3, order="F")
+        _, P, _, _ = neohookean(F, Fpe, theta[e], A=False, mixed=False)
+        B = np.zeros((9, 12))  # vec(dF)[i + 3 j] = du_b[i] g_b[j]
+        for b in range(4):
+            for i in range(3):
+                for j in range(3):
+                    B[i + 3 * j, 3 * b + i] = G[b, j]
+        fe = we * B.T @ P.reshape(-1, order="F")
+        for b in range(4):
+            f[3 * ids[b]:3 * ids[b] + 3] += fe[3 * b:3 * b + 3]
+        if w is None and dFp is None:
+            continue
+        Je = we * B.T @ d2W_dF_dFp(F, Fpe, theta[e])
+        if jt is not None:
+            jt[e] = Je.T @ wv[ids].reshape(-1)
+        if jv is not None:
+            de = Je @ dv[e]
+            for b in range(4):
+                jv[3 * ids[b]:3 * ids[b] + 3] += de[3 * b:3 * b + 3]
+    return f, jt, jv

@@ -96,3 +96,66 @@ def test_rejected_plastic_parts_fail_evaluations(torch):
             dev.eval(u, True, "exact")
         with pytest.raises(ValueError):
             dev.max_feasible_step(u, u)
+
+
+def colmajor(Fp):
+    return np.ascontiguousarray(np.asarray(Fp).transpose(0, 2, 1).reshape(len(Fp), 9))
+
+
+def test_force_fp_response(torch, case, oracle_mod):
+    """VolumetricFemForceFpResponse (volumetric_fem_force_response.hpp:81-179, neo_hookean.cpp:103-128): value,
+    forward and transpose products in the per-tet column-major vec(Fp) against the oracle, with the handle's own Fp
+    and with per-call plastic parts that differ from it."""
+    X, tets, theta, Fp, u, dev, ref = case
+    T = tets.shape[0]
+    x = torch.from_numpy(u).cuda()
+    _, g, _ = ref.eval(u, True)
+    assert rel(dev.force_fp_device(x).cpu().numpy(), g) <= TOL
+    assert rel(dev.force_fp_device(x, torch.from_numpy(colmajor(Fp)).cuda()).cpu().numpy(), g) <= TOL
+    rng = np.random.default_rng(8)
+    Fp2 = np.eye(3)[None] + 0.1 * rng.normal(size=(T, 3, 3))
+    assert np.all(np.linalg.det(Fp2) > 0)
+    fp2 = colmajor(Fp2)
+    w = rng.normal(size=u.size)
+    d = rng.normal(size=(T, 9))
+    fr, jtr, jvr = oracle_mod.force_fp_products(tets, X, theta, u, fp2, w=w, dFp=d)
+    fpd = torch.from_numpy(fp2).cuda()
+    f = dev.force_fp_device(x, fpd).cpu().numpy()
+    jt = dev.force_vjp_fp_device(x, torch.from_numpy(w).cuda(), fpd).cpu().numpy()
+    jv = dev.force_jvp_fp_device(x, torch.from_numpy(d).cuda(), fpd).cpu().numpy()
+    print(f"force_fp: df {rel(f, fr):.2e} dJtw {rel(jt, jtr):.2e} dJd {rel(jv, jvr):.2e}", flush=True)
+    assert rel(f, fr) <= TOL and rel(jt, jtr) <= TOL and rel(jv, jvr) <= TOL
+    # <w, J d> = <J^T w, d> between the two device kernels.
+    assert abs(w @ jv - np.sum(jt * d)) <= 1e-12 * (np.abs(w) @ np.abs(jv))
+    # A plain handle (every Fp = I) takes per-call plastic parts the same way.
+    plain = VolumetricFemEnergy(len(X), tets, X, theta)
+    assert rel(plain.force_fp_device(x, fpd).cpu().numpy(), fr) <= TOL
+    assert rel(plain.force_fp_device(x).cpu().numpy(), plain.eval(u, True)[1]) <= TOL
+
+
+def test_force_fp_response_errors_and_unused_vertex(torch, oracle_mod):
+    X, tets = kuhn_grid(2)
+    X = np.vstack([X, [[5.0, 5.0, 5.0]]])  # a vertex no tet uses: zero force and zero products
+    T = tets.shape[0]
+    lam, mu = lame_parameters(1e5, 0.4)
+    theta = np.tile([lam, mu], (T, 1))
+    rng = np.random.default_rng(9)
+    u = 0.01 * rng.normal(size=X.size)
+    dev = VolumetricFemEnergy(len(X), tets, X, theta)
+    Fp = np.eye(3)[None] + 0.1 * rng.normal(size=(T, 3, 3))
+    fp = colmajor(Fp)
+    x = torch.from_numpy(u).cuda()
+    d = rng.normal(size=(T, 9))
+    f = dev.force_fp_device(x, torch.from_numpy(fp).cuda()).cpu().numpy()
+    jv = dev.force_jvp_fp_device(x, torch.from_numpy(d).cuda(), torch.from_numpy(fp).cuda()).cpu().numpy()
+    fr, _, jvr = oracle_mod.force_fp_products(tets, X, theta, u, fp, dFp=d)
+    assert rel(f, fr) <= TOL and rel(jv, jvr) <= TOL
+    assert np.all(f[-3:] == 0.0) and np.all(jv[-3:] == 0.0)
+    bad = fp.copy()
+    bad[5] = (-np.eye(3)).ravel()  # det Fp < 0: MultiplicativeKinematics' domain error
+    with pytest.raises(ValueError):
+        dev.force_fp_device(x, torch.from_numpy(bad).cuda())
+    with pytest.raises(ValueError):
+        dev.force_vjp_fp_device(x, x, torch.from_numpy(bad).cuda())
+    # The handle recovers after the error.
+    assert rel(dev.force_fp_device(x, torch.from_numpy(fp).cuda()).cpu().numpy(), fr) <= TOL
