@@ -734,7 +734,8 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
 #pragma unroll
     for (int s = 0; s < LPV; ++s) {
         if (owner && sub == s) {
-            if (KIND >= 0) {
+            // A vertex no tet uses has an empty row (and zero sums): its `row` is the next row's start.
+            if (KIND >= 0 && rowlen > 0) {
                 double* blk = row + 3 * self;
                 blk[0] += ds[0];
                 blk[1] += ds[1];

@@ -237,6 +237,25 @@ def test_high_degree_vertex_uses_wide_incidences(oracle_mod):
         dev.eval_values(x, True, "exact")
 
 
+@pytest.mark.parametrize("num_vertices", [0, 4])
+def test_empty_mesh(gpu, num_vertices):
+    """No tets (the reference's zero-element meshes): E = 0, a zero gradient, an empty Hessian, zero products."""
+    torch = gpu
+    X = np.eye(4, 3)[:num_vertices]
+    dev = VolumetricFemEnergy(num_vertices, np.zeros((0, 4), dtype=np.int32), X, np.zeros((0, 2)))
+    assert dev.nnz == 0 and dev.num_dofs == 3 * num_vertices
+    x = np.full(dev.num_dofs, 0.1)
+    for kind in ("exact", "projected"):
+        print(f"  V={num_vertices} kind={kind}", flush=True)
+        v, g, H = dev.eval_values(x, True, kind)
+        assert v == 0.0 and g.shape == (dev.num_dofs,) and np.all(g == 0.0) and H.size == 0
+    xd = torch.from_numpy(x).cuda()
+    fp = torch.zeros(0, dtype=torch.float64, device="cuda")
+    assert torch.all(dev.force_fp_device(xd, fp) == 0.0) and dev.force_fp_device(xd, fp).numel() == dev.num_dofs
+    assert dev.force_vjp_fp_device(xd, xd, fp).shape == (0, 9)
+    assert dev.max_feasible_step(x, x) == 1.0
+
+
 def test_error_semantics_and_recovery():
     w = workloads.plain(4, 1e5, 0.4, 0.05, seed=1)
     dev = VolumetricFemEnergy(w.num_vertices, w.tets, w.rest, w.theta)
