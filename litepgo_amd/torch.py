@@ -17,6 +17,8 @@ Derivatives and the kernels that compute them (lpg_kernels.hip):
   d(u . dE/dtheta)/dx = (df/dtheta) u                    (k_vertex_gather, force JVP in theta)
   d(dE/dtheta)/dtheta = 0                                (psi is linear in theta)
   (lambda, mu)(E, nu) and its reverse product            (k_lame, k_lame_vjp)
+  d(w . f)/dFp_e   = (df/dFp_e)^T w                      (k_fp_vjp; `force_fp`, x and theta fixed)
+  (df/dFp) Fp_dot                                        (k_fp_elem + k_elem_gather)
 A backward pass is one library call (lpg_tet_energy_vjp_device: both cotangents, both outputs, one status
 read-back); the energy stays on the device from the assembly to the returned tensor.
 """
@@ -25,7 +27,7 @@ from __future__ import annotations
 import torch
 from torch.autograd.function import once_differentiable
 
-__all__ = ["energy", "force", "energy_and_force", "lame"]
+__all__ = ["energy", "force", "energy_and_force", "force_fp", "lame"]
 
 
 class _Lame(torch.autograd.Function):
@@ -214,6 +216,51 @@ class _Energy(torch.autograd.Function):
             d = _ParamGrad.apply(x, theta, p)
             out = out + (d.reshape(-1) * _out(theta_dot, d).reshape(-1)).sum()
         return out
+
+
+class _ForceFp(torch.autograd.Function):
+    """f(Fp) = dE/dx at fixed x and theta (VolumetricFemForceFpResponse); Fp is (T, 9) column-major vec per tet."""
+
+    @staticmethod
+    def forward(ctx, Fp, x, theta, potential):
+        p = potential
+        xd = _device_input(x, "x", p.num_dofs)
+        out = p.force_fp_device(xd, _device_input(Fp, "Fp", 9 * p.num_tets), theta=_theta_input(p, theta))
+        ctx.save_for_backward(Fp, x, theta)
+        ctx.save_for_forward(Fp, x, theta)
+        ctx.potential = p
+        return _out(out, x)
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, w):
+        Fp, x, theta = ctx.saved_tensors
+        if not ctx.needs_input_grad[0]:
+            return None, None, None, None
+        p = ctx.potential
+        g = p.force_vjp_fp_device(_device_input(x, "x", p.num_dofs), _device_input(w, "w", p.num_dofs),
+                                  _device_input(Fp, "Fp", 9 * p.num_tets), theta=_theta_input(p, theta))
+        return _out(g, Fp).reshape(Fp.shape), None, None, None
+
+    @staticmethod
+    def jvp(ctx, Fp_dot, _x_dot, _theta_dot, _potential):
+        Fp, x, theta = ctx.saved_tensors
+        p = ctx.potential
+        if Fp_dot is None:
+            return torch.zeros(p.num_dofs, dtype=torch.float64, device=x.device)
+        out = p.force_jvp_fp_device(_device_input(x, "x", p.num_dofs),
+                                    _device_input(Fp_dot, "Fp_dot", 9 * p.num_tets),
+                                    _device_input(Fp, "Fp", 9 * p.num_tets), theta=_theta_input(p, theta))
+        return _out(out, x)
+
+
+def force_fp(potential, Fp, x, theta=None):
+    """The force as a function of the plastic parts, f(Fp) = dE/dx at fixed x and theta (the reference's
+    VolumetricFemForceFpResponse with an ElementDofMap of width 9): Fp is (T, 9), column-major vec(Fp) per tet, and
+    the only differentiable input; reverse mode gives (df/dFp)^T w, forward mode (df/dFp) Fp_dot."""
+    if not isinstance(Fp, torch.Tensor) or not isinstance(x, torch.Tensor):
+        raise TypeError("expected torch.Tensor inputs")
+    return _ForceFp.apply(Fp, x, theta, potential)
 
 
 def energy(potential, x, theta=None):

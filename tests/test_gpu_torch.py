@@ -182,3 +182,34 @@ def test_fused_vjp_matches_separate_products(torch):
     Ed, _, _ = pot.eval_device_energy(x, False, None, theta=theta)
     v, _, _ = pot.eval_device(x, False, None, theta=theta)
     assert float(Ed[0]) == v
+
+
+def test_force_fp_reverse_and_forward_modes(torch):
+    """force_fp: autograd's reverse product against central differences of the force in Fp, and forward-mode AD
+    against the same differences (the reference's Fp force response, tests/python/test_fp_force_response.py,
+    differentiates the same map)."""
+    from litepgo_amd import VolumetricFemEnergy, kuhn_grid, lame_parameters
+    from litepgo_amd import torch as lt
+
+    X, tets = kuhn_grid(3)
+    T = tets.shape[0]
+    rng = np.random.default_rng(11)
+    lam, mu = lame_parameters(1e5, 0.4)
+    pot = VolumetricFemEnergy(len(X), tets, X, np.tile([lam, mu], (T, 1)))
+    x = torch.from_numpy(0.02 * rng.normal(size=X.size)).cuda()
+    Fp0 = np.eye(3)[None] + 0.1 * rng.normal(size=(T, 3, 3))
+    Fp = torch.from_numpy(np.ascontiguousarray(Fp0.transpose(0, 2, 1).reshape(T, 9))).cuda().requires_grad_(True)
+    f = lt.force_fp(pot, Fp, x)
+    w = torch.from_numpy(rng.normal(size=pot.num_dofs)).cuda()
+    (gFp,) = torch.autograd.grad((w * f).sum(), (Fp,))
+    d = torch.from_numpy(rng.normal(size=(T, 9))).cuda()
+    F0 = Fp.detach()
+    fd = central(lambda h: lt.force_fp(pot, F0 + h * d, x), STEP)
+    ad = float((gFp * d).sum())
+    assert abs(float((w * fd).sum()) - ad) <= 1e-6 * (abs(ad) + float(torch.linalg.vector_norm(gFp)))
+    import torch.autograd.forward_ad as fwAD
+
+    with fwAD.dual_level():
+        tangent = fwAD.unpack_dual(lt.force_fp(pot, fwAD.make_dual(F0, d), x)).tangent
+    scale = float(torch.abs(fd).max())
+    assert float(torch.abs(tangent - fd).max()) <= 1e-6 * scale
