@@ -188,6 +188,43 @@ def bench_differentiation(steps, warmup):
             "forward_tets_per_s": T / (f_ms * 1e-3), "backward_tets_per_s": T / (b_ms * 1e-3)}
 
 
+def bench_side_products(dev, w, steps, warmup):
+    """The operator's other entry points on the headline workload, wall-clock medians with the host synchronised on
+    both sides (each call ends with its status read-back): the feasible step along a random direction, and the
+    force response to per-call plastic parts (Fp = I + 0.05 N(0, 1) per tet): value, JVP and VJP."""
+    import numpy as np
+    import torch
+
+    rng = np.random.default_rng(3)
+    T = dev.num_tets
+    x = torch.from_numpy(w.u).cuda()
+    dx = torch.from_numpy(rng.normal(scale=0.05, size=dev.num_dofs)).cuda()
+    Fp = np.eye(3)[None] + 0.05 * rng.normal(size=(T, 3, 3))
+    fp = torch.from_numpy(np.ascontiguousarray(Fp.transpose(0, 2, 1).reshape(T, 9))).cuda()
+    dfp = torch.from_numpy(rng.normal(size=(T, 9))).cuda()
+    wv = torch.from_numpy(rng.normal(size=dev.num_dofs)).cuda()
+    calls = {
+        "feasible_step": lambda: dev.max_feasible_step_device(x, dx),
+        "force_fp": lambda: dev.force_fp_device(x, fp),
+        "force_jvp_fp": lambda: dev.force_jvp_fp_device(x, dfp, fp),
+        "force_vjp_fp": lambda: dev.force_vjp_fp_device(x, wv, fp),
+    }
+    out = {"workload": "plain 64^3 Kuhn grid (as the headline); Fp = I + 0.05 N(0, 1) per tet"}
+    for name, call in calls.items():
+        for _ in range(warmup):
+            call()
+        times = []
+        for _ in range(steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            call()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        ms = statistics.median(times) * 1e3
+        out[name] = {"ms": ms, "tets_per_s": T / (ms * 1e-3)}
+    return out
+
+
 def bench_equilibrium(batch):
     """Equilibrium sensitivity (cantilever, x = 0 face fixed, gravity 9.8 along -z, density 1000, per-tet E
     log-normal(ln 1e5, 0.5), nu 0.45): device Newton with Jacobi-PCG to |g|_inf < 1e-8 from rest, the same for the
@@ -246,6 +283,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-mesh batched comparison")
     ap.add_argument("--no-differentiation", action="store_true", help="skip the 32^3 forward/backward timing")
     ap.add_argument("--no-equilibrium", action="store_true", help="skip the 48^3 equilibrium-sensitivity solve")
+    ap.add_argument("--no-side", action="store_true", help="skip the feasible-step and plastic-response timings")
     ap.add_argument("--equilibrium-batch", action="store_true",
                     help="also solve the 8-beam (40..48 cells per axis) equilibrium batch")
     ap.add_argument("--probe-copy-mb", type=int, default=0,
@@ -331,6 +369,8 @@ def main():
                          "note": "lpg_tet_energy_eval with host x, g and H (PCIe-inclusive); not the headline"},
         "exact": exact,
     }
+    if not args.no_side:
+        result["side_products"] = bench_side_products(dev, w, args.steps, args.warmup)
     if not args.no_batched:
         result["batched"] = bench_batched(args.steps, args.warmup)
     if not args.no_differentiation:

@@ -1428,29 +1428,43 @@ __global__ __launch_bounds__(256) void k_fp_elem(FpParams p) {
         atomicOr(p.status, flags);
 }
 
-// out_v = sum over the incidences of v, in list order, of the element values of its slot (12 per tet).
-__global__ __launch_bounds__(256) void k_elem_gather(const int32_t* __restrict__ inc_ptr,
-                                                     const int32_t* __restrict__ inc_tet, int inc_stride,
-                                                     const int32_t* __restrict__ conn,
+// out_v = sum over the incidences of v of the element values of its slot (12 per tet).  Eight lanes share a vertex
+// and stride over its list (the record holds the slot, so the connectivity is not read); their partial sums are
+// combined by a fixed xor tree, so the result does not depend on timing.
+constexpr int kElemGatherLanes = 8;
+
+template <bool WIDE>
+__global__ __launch_bounds__(256) void k_elem_gather(const int32_t* __restrict__ inc_ptr, const void* inc_,
                                                      const double* __restrict__ elem, double* __restrict__ out,
                                                      int num_vertices) {
-    const int v = blockIdx.x * 256 + threadIdx.x;
+    using Inc = typename IncT<WIDE>::type;
+    const Inc* __restrict__ inc = static_cast<const Inc*>(inc_);
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int v = gid / kElemGatherLanes;
+    const int sub = gid % kElemGatherLanes;
+    // A group is either wholly past the end or wholly active, so the shuffles below read active lanes only.
     if (v >= num_vertices)
         return;
     double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-    for (int k = inc_ptr[v]; k < inc_ptr[v + 1]; ++k) {
-        const int e = *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(inc_tet) +
-                                                         static_cast<size_t>(k) * inc_stride);
-        const int4 c = reinterpret_cast<const int4*>(conn)[e];
-        const int a = (c.x == v) ? 0 : (c.y == v) ? 1 : (c.z == v) ? 2 : 3;
-        const double* src = elem + 12 * static_cast<size_t>(e) + 3 * a;
+    const int ke = inc_ptr[v + 1];
+    for (int k = inc_ptr[v] + sub; k < ke; k += kElemGatherLanes) {
+        const Inc r = inc[k];
+        const double* src = elem + 12 * static_cast<size_t>(r.tet) + 3 * IncT<WIDE>::slot(r);
         y0 += src[0];
         y1 += src[1];
         y2 += src[2];
     }
-    out[3 * static_cast<size_t>(v)] = y0;
-    out[3 * static_cast<size_t>(v) + 1] = y1;
-    out[3 * static_cast<size_t>(v) + 2] = y2;
+#pragma unroll
+    for (int m = kElemGatherLanes / 2; m > 0; m >>= 1) {
+        y0 += __shfl_xor(y0, m, kElemGatherLanes);
+        y1 += __shfl_xor(y1, m, kElemGatherLanes);
+        y2 += __shfl_xor(y2, m, kElemGatherLanes);
+    }
+    if (sub == 0) {
+        out[3 * static_cast<size_t>(v)] = y0;
+        out[3 * static_cast<size_t>(v) + 1] = y1;
+        out[3 * static_cast<size_t>(v) + 2] = y2;
+    }
 }
 
 // Vertex-gather products over each vertex's incidence list:
@@ -1649,17 +1663,14 @@ __device__ double det_feasible_step(const double A[9], const double dA[9]) {
     const auto tolerance = [&](double t) {
         return 64.0 * kEps * (((fabs(c3) * t + fabs(c2)) * t + fabs(c1)) * t + c0);
     };
-    // Knots: 0, 1 and the stationary points inside (0, 1), in increasing order.
-    double knot[4] = {0.0, 1.0, 2.0, 2.0};
-    int count = 2;
-    const auto add_knot = [&](double t) {
-        if (t > 0.0 && t < 1.0)
-            knot[count++] = t;
-    };
+    // Stationary points of the cubic inside (0, 1), in increasing order (2 marks an absent one); with 0 and 1 they
+    // split [0, 1] into the pieces [0, k1], [k1, k2], [k2, 1].  Registers only (no dynamically indexed array).
+    const auto inside = [](double t) { return t > 0.0 && t < 1.0 ? t : 2.0; };
+    double r1 = 2.0, r2 = 2.0;
     const double qa = 3.0 * c3, qb = 2.0 * c2, qc = c1;
     if (qa == 0.0) {
         if (qb != 0.0)
-            add_knot(-qc / qb);
+            r1 = inside(-qc / qb);
     } else {
         double disc = fma(qb, qb, -4.0 * qa * qc);
         const double err = 8.0 * kEps * (qb * qb + fabs(4.0 * qa * qc));
@@ -1667,37 +1678,42 @@ __device__ double det_feasible_step(const double A[9], const double dA[9]) {
             disc = fmax(0.0, disc);
             const double q = -0.5 * (qb + copysign(sqrt(disc), qb));
             if (q == 0.0) {
-                add_knot(-qb / (2.0 * qa));
+                r1 = inside(-qb / (2.0 * qa));
             } else {
-                add_knot(q / qa);
-                add_knot(qc / q);
+                r1 = inside(q / qa);
+                r2 = inside(qc / q);
             }
         }
     }
-    // Insertion sort of at most four values.
-    for (int i = 1; i < count; ++i)
-        for (int j = i; j > 0 && knot[j] < knot[j - 1]; --j) {
-            const double tmp = knot[j];
-            knot[j] = knot[j - 1];
-            knot[j - 1] = tmp;
-        }
-    for (int i = 1; i < count; ++i) {
-        double lo = knot[i - 1], hi = knot[i];
-        if (value(hi) > tolerance(hi))
+    const double k1 = fmin(r1, r2), k2 = fmax(r1, r2);
+    // The first piece whose right end is not safely positive is bisected to the last safe t.
+    double lo = 0.0, hi = 1.0;
+    bool found = false;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double end = i == 0 ? k1 : (i == 1 ? k2 : 1.0);
+        if (found || end > 1.0)
             continue;
-        // At most ~1100 halvings reach mid == lo or mid == hi for doubles in [0, 1].
-        for (;;) {
-            const double mid = lo + (hi - lo) / 2.0;
-            if (mid == lo || mid == hi)
-                break;
-            if (value(mid) > tolerance(mid))
-                lo = mid;
-            else
-                hi = mid;
+        if (value(end) > tolerance(end)) {
+            lo = end;
+        } else {
+            hi = end;
+            found = true;
         }
-        return 0.9 * lo;
     }
-    return 1.0;
+    if (!found)
+        return 1.0;
+    // At most ~1100 halvings reach mid == lo or mid == hi for doubles in [0, 1].
+    for (;;) {
+        const double mid = lo + (hi - lo) / 2.0;
+        if (mid == lo || mid == hi)
+            break;
+        if (value(mid) > tolerance(mid))
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return 0.9 * lo;
 }
 
 struct StepParams {
@@ -1785,8 +1801,14 @@ __global__ __launch_bounds__(256) void k_feasible_step(StepParams p) {
 #pragma unroll
         for (int off = kWave / 2; off > 0; off >>= 1)
             m = fmin(m, __shfl_xor(m, off));
-        if ((threadIdx.x % kWave) == 0 && m < 1.0)
-            atomicMin(p.alpha_bits + seg0, static_cast<unsigned long long>(__double_as_longlong(m)));
+        // Positive doubles order as their bit patterns.  A wave whose minimum does not lower the segment's current
+        // bound skips the atomic: one L2 address takes every wave's update otherwise.  (A stale read only costs an
+        // unneeded atomic.)
+        if ((threadIdx.x % kWave) == 0 && m < 1.0) {
+            const unsigned long long mb = static_cast<unsigned long long>(__double_as_longlong(m));
+            if (mb < __hip_atomic_load(p.alpha_bits + seg0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMin(p.alpha_bits + seg0, mb);
+        }
     } else if (e < p.num_tets && alpha < 1.0) {
         atomicMin(p.alpha_bits + seg, static_cast<unsigned long long>(__double_as_longlong(alpha)));
     }
@@ -2047,10 +2069,14 @@ void launch_force_fp(const DeviceMesh& m, const double* x, const double* theta, 
     if (m.num_vertices == 0)
         return;
     // Vertices with no incidences (every vertex when there are no tets) get zeros.
-    hipLaunchKernelGGL(k_elem_gather, dim3((m.num_vertices + 255) / 256), dim3(256), 0, stream, m.inc_ptr,
-                       static_cast<const int32_t*>(m.inc),
-                       m.wide ? static_cast<int>(sizeof(IncWide)) : static_cast<int>(sizeof(IncNarrow)), m.conn,
-                       scratch, out, m.num_vertices);
+    const long long threads = static_cast<long long>(m.num_vertices) * kElemGatherLanes;
+    const dim3 ggrid(static_cast<unsigned>((threads + 255) / 256));
+    if (m.wide)
+        hipLaunchKernelGGL(k_elem_gather<true>, ggrid, dim3(256), 0, stream, m.inc_ptr, m.inc, scratch, out,
+                           m.num_vertices);
+    else
+        hipLaunchKernelGGL(k_elem_gather<false>, ggrid, dim3(256), 0, stream, m.inc_ptr, m.inc, scratch, out,
+                           m.num_vertices);
     check_launch("k_elem_gather");
 }
 
