@@ -159,3 +159,26 @@ def test_force_fp_response_errors_and_unused_vertex(torch, oracle_mod):
         dev.force_vjp_fp_device(x, x, torch.from_numpy(bad).cuda())
     # The handle recovers after the error.
     assert rel(dev.force_fp_device(x, torch.from_numpy(fp).cuda()).cpu().numpy(), fr) <= TOL
+
+
+def test_force_fp_fits_a_plastic_scale(torch):
+    """The reference's Gauss-Newton fit of a shared plastic scale (tests/python/test_fp_force_response.py:44-66): the
+    force at Fp = 1.05 I is the target, and Gauss-Newton on the scalar s (residual f(s I) - target, Jacobian the
+    forward product along I in every tet) recovers s = 1.05 from 1."""
+    X, tets = kuhn_grid(3)
+    T = tets.shape[0]
+    lam, mu = lame_parameters(1e5, 0.4)
+    dev = VolumetricFemEnergy(len(X), tets, X, np.tile([lam, mu], (T, 1)))
+    rng = np.random.default_rng(12)
+    x = torch.from_numpy(0.02 * rng.normal(size=X.size)).cuda()
+    direction = torch.from_numpy(np.tile(np.eye(3).ravel(order="F"), (T, 1))).cuda()
+    target = dev.force_fp_device(x, 1.05 * direction)
+    s = 1.0
+    for _ in range(20):
+        r = dev.force_fp_device(x, s * direction) - target
+        Jd = dev.force_jvp_fp_device(x, direction, s * direction)
+        grad = float(torch.dot(Jd, r))
+        if abs(grad) <= 1e-10 * float(torch.linalg.vector_norm(Jd)) * float(torch.linalg.vector_norm(target)):
+            break
+        s -= grad / float(torch.dot(Jd, Jd))
+    assert abs(s - 1.05) <= 1e-8
