@@ -291,8 +291,9 @@ def main():
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the sharded design is described "
-              "in DESIGN.md.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the operator runs on one device, "
+              "and a multi-GPU run would give each rank its own meshes or its own share of one mesh.",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
