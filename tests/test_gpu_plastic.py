@@ -127,6 +127,13 @@ def test_force_fp_response(torch, case, oracle_mod):
     assert rel(f, fr) <= TOL and rel(jt, jtr) <= TOL and rel(jv, jvr) <= TOL
     # <w, J d> = <J^T w, d> between the two device kernels.
     assert abs(w @ jv - np.sum(jt * d)) <= 1e-12 * (np.abs(w) @ np.abs(jv))
+    # A per-call theta overrides the snapshot in all three products.
+    theta2 = theta * np.array([1.3, 0.8])
+    td = torch.from_numpy(theta2).cuda()
+    fr2, jtr2, jvr2 = oracle_mod.force_fp_products(tets, X, theta2, u, fp2, w=w, dFp=d)
+    assert rel(dev.force_fp_device(x, fpd, theta=td).cpu().numpy(), fr2) <= TOL
+    assert rel(dev.force_vjp_fp_device(x, torch.from_numpy(w).cuda(), fpd, theta=td).cpu().numpy(), jtr2) <= TOL
+    assert rel(dev.force_jvp_fp_device(x, torch.from_numpy(d).cuda(), fpd, theta=td).cpu().numpy(), jvr2) <= TOL
     # A plain handle (every Fp = I) takes per-call plastic parts the same way.
     plain = VolumetricFemEnergy(len(X), tets, X, theta)
     assert rel(plain.force_fp_device(x, fpd).cpu().numpy(), fr) <= TOL
