@@ -237,6 +237,35 @@ def test_high_degree_vertex_uses_wide_incidences(oracle_mod):
         dev.eval_values(x, True, "exact")
 
 
+def test_high_degree_vertex_force_fp_and_products(oracle_mod, gpu):
+    """The wide-incidence fan through the vertex-gather products: the plastic force response (its gather reads the
+    wide records) and the Hessian-vector product, against the oracle."""
+    torch = gpu
+    n = 120
+    X, tets = fan_mesh(n, 4)
+    theta = np.tile([[1e4, 2e4]], (n, 1))
+    dev = VolumetricFemEnergy(len(X), tets, X, theta)
+    assert dev.max_degree == 3 * n + 1
+    rng = np.random.default_rng(6)
+    u = rng.uniform(-0.01, 0.01, X.size)
+    Fp = np.eye(3)[None] + 0.1 * rng.normal(size=(n, 3, 3))
+    fp = np.ascontiguousarray(Fp.transpose(0, 2, 1).reshape(n, 9))
+    d = rng.normal(size=(n, 9))
+    fr, _, jvr = oracle_mod.force_fp_products(tets, X, theta, u, fp, dFp=d)
+    x = torch.from_numpy(u).cuda()
+    fpd = torch.from_numpy(fp).cuda()
+    assert rel(dev.force_fp_device(x, fpd).cpu().numpy(), fr) <= TOL
+    assert rel(dev.force_jvp_fp_device(x, torch.from_numpy(d).cuda(), fpd).cpu().numpy(), jvr) <= TOL
+    ref = oracle_mod.OracleEnergy(len(X), tets, X, theta)
+    _, _, Hr = ref.eval(u, False, "exact")
+    outer, inner = ref.pattern()
+    import scipy.sparse as sp
+
+    v = rng.normal(size=u.size)
+    Hv = sp.csc_matrix((Hr, inner, outer), shape=(ref.num_dofs, ref.num_dofs)) @ v
+    assert rel(dev.hvp_device(x, torch.from_numpy(v).cuda()).cpu().numpy(), Hv) <= TOL
+
+
 @pytest.mark.parametrize("num_vertices", [0, 4])
 def test_empty_mesh(gpu, num_vertices):
     """No tets (the reference's zero-element meshes): E = 0, a zero gradient, an empty Hessian, zero products."""
