@@ -189,3 +189,26 @@ def test_force_fp_fits_a_plastic_scale(torch):
             break
         s -= grad / float(torch.dot(Jd, Jd))
     assert abs(s - 1.05) <= 1e-8
+
+
+def test_force_fp_closed_form_dilation(torch):
+    """One unit tet at F = a I with Fp = b I (the reference's analytic case, neo_hookean_gtest.cpp:270-281): the force
+    is f_b = w b^2 (mu a/b + (3 lambda log(a/b) - mu) b/a) g_b, the forward product along vec(I) is
+    w (mu a + b^2/a (9 lambda log(a/b) - 3 mu - 3 lambda)) g_b, and the transpose product of any w_f dotted with
+    vec(I) equals w_f . (J vec(I)); w = 1/6, g_0 = -(1, 1, 1), g_i = e_i."""
+    lam, mu, a, b = 2.0, 3.0, 1.3, 0.8
+    X = np.vstack([np.zeros(3), np.eye(3)])
+    dev = VolumetricFemEnergy(4, np.array([[0, 1, 2, 3]], dtype=np.int32), X, np.array([[lam, mu]]))
+    G = np.vstack([-np.ones(3), np.eye(3)])
+    x = torch.from_numpy(((a - 1.0) * X).ravel()).cuda()
+    vecI = np.eye(3).ravel(order="F")[None]
+    fp = torch.from_numpy(b * vecI).cuda()
+    f = dev.force_fp_device(x, fp).cpu().numpy()
+    f_expected = (b * b * (mu * a / b + (3 * lam * np.log(a / b) - mu) * b / a) / 6.0) * G.ravel()
+    np.testing.assert_allclose(f, f_expected, rtol=1e-13, atol=1e-14)
+    jv = dev.force_jvp_fp_device(x, torch.from_numpy(vecI).cuda(), fp).cpu().numpy()
+    scalar = mu * a + b * b / a * (9 * lam * np.log(a / b) - 3 * mu - 3 * lam)
+    np.testing.assert_allclose(jv, scalar / 6.0 * G.ravel(), rtol=1e-13, atol=1e-14)
+    wf = np.random.default_rng(13).normal(size=12)
+    jt = dev.force_vjp_fp_device(x, torch.from_numpy(wf).cuda(), fp).cpu().numpy()
+    assert abs(float(jt.ravel() @ vecI.ravel()) - float(wf @ jv)) <= 1e-13 * np.abs(wf) @ np.abs(jv)

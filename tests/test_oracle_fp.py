@@ -93,3 +93,31 @@ def test_planted_mistake_fails_the_difference_check(oracle_mod, mistake):
     fd = point_fd(oracle_mod, F, Fp, theta)
     np.testing.assert_allclose(good, fd, rtol=1e-6, atol=1e-7)
     assert not np.allclose(bad, fd, rtol=1e-6, atol=1e-7)
+
+
+def test_reference_known_answers(oracle_mod):
+    """tests/cpp/fem/material/volumetric/neo_hookean_gtest.cpp:270-300 of the reference: at F = Fp = I the mixed
+    derivative is minus the tangent; at F = a I, Fp = b I it maps vec(I) to (mu a + b^2/a (9 lambda log(a/b)
+    - 3 mu - 3 lambda)) vec(I); at its fixed (F, Fp) it matches central differences of P (1e-8); and a zero Fp or an
+    inverted F is a domain error, a non-finite Fp or mu = 0 an invalid argument."""
+    I = np.eye(3)
+    theta = np.array([2.0, 3.0])
+    A = oracle_mod.neohookean(I, I, theta, P=False, mixed=False)[2]
+    np.testing.assert_allclose(oracle_mod.d2W_dF_dFp(I, I, theta), -A, rtol=0, atol=1e-14 * np.abs(A).max())
+    a, b = 1.3, 0.8
+    B = oracle_mod.d2W_dF_dFp(a * I, b * I, theta)
+    expected = theta[1] * a + b * b / a * (9 * theta[0] * np.log(a / b) - 3 * theta[1] - 3 * theta[0])
+    np.testing.assert_allclose(B @ I.ravel(order="F"), expected * I.ravel(order="F"), rtol=1e-13, atol=1e-13)
+    F = np.array([[1.2, .2, -.1], [.05, .9, .1], [.02, -.04, 1.1]])
+    Fp = np.array([[.9, -.1, .03], [.04, 1.2, .08], [-.02, .05, 1.1]])
+    M = oracle_mod.d2W_dF_dFp(F, Fp, theta)
+    fd = point_fd(oracle_mod, F, Fp, theta)
+    for c in range(9):
+        assert np.linalg.norm(M[:, c] - fd[:, c]) <= 1e-8 * max(np.linalg.norm(fd[:, c]), np.linalg.norm(M[:, c]))
+    nan_Fp = Fp.copy()
+    nan_Fp[0, 0] = np.nan
+    for args, code in (((F, 1e-300 * I, theta), 2), ((-F, Fp, theta), 2), ((F, Fp, np.array([2.0, 0.0])), 1),
+                       ((F, nan_Fp, theta), 1)):
+        with pytest.raises(oracle_mod.OracleError) as err:
+            oracle_mod.d2W_dF_dFp(*args)
+        assert err.value.code == code
