@@ -39,3 +39,14 @@ def test_bridge_argument_errors():
         lt.force(None, [0.0])
     with pytest.raises(TypeError):
         lt.energy_and_force(None, 1.0)
+    with pytest.raises(TypeError):
+        lt.force_fp(None, np.zeros(9), torch.zeros(3, dtype=torch.float64))
+
+
+def test_force_fp_entry_points_reject_a_null_handle():
+    lib = _native.lib()
+    assert lib.lpg_tet_energy_force_fp_device(None, None, None, None, None, None) == _native.LPG_INVALID_ARGUMENT
+    assert lib.lpg_tet_energy_force_vjp_fp_device(None, None, None, None, None, None, None) == \
+        _native.LPG_INVALID_ARGUMENT
+    assert lib.lpg_tet_energy_force_jvp_fp_device(None, None, None, None, None, None, None) == \
+        _native.LPG_INVALID_ARGUMENT
