@@ -90,7 +90,7 @@ lpg_tet_energy::~lpg_tet_energy() {
     graph.reset();
     for (lpg::DeviceBuffer* b : {&d_Xr, &d_conn, &d_theta, &d_inc_ptr, &d_inc, &d_nbr_ptr, &d_nbr, &d_self_pos, &d_row_ptr, &d_scalars, &d_vertex_E, &d_chunk_begin,
                                  &d_chunk_sum, &d_seg_chunk, &d_proj_slot, &d_proj_list, &d_proj_H, &d_x, &d_g, &d_H,
-                                 &d_dx, &d_seg_tbeg, &d_seg_all, &d_seg_energy, &d_fp, &d_fp_elem,
+                                 &d_dx, &d_seg_tbeg, &d_seg_all, &d_seg_energy, &d_fp, &d_fp_elem, &d_tet_inc,
                                  &plan.d_tile_vbeg, &plan.d_normal_ids, &plan.d_oversized_ids})
         b->release();
     if (pinned)
@@ -572,11 +572,36 @@ lpg_status lpg_tet_energy_force_jvp_theta_device(lpg_tet_energy* h, const double
                                 });
 }
 
+namespace lpg {
+namespace {
+// Scratch of the plastic force response: one 3-vector per incidence, written by tet through the incidence index of
+// each (tet, slot), then summed per vertex over its contiguous incidence range.
+double* fp_scratch(lpg_tet_energy* h) {
+    const auto& s = h->setup;
+    const size_t T = static_cast<size_t>(s.num_tets);
+    if (T > 0 && h->d_tet_inc.bytes < 4 * T * sizeof(int32_t)) {
+        std::vector<int32_t> tet_inc(4 * T);
+        for (int32_t v = 0; v < s.num_vertices; ++v)
+            for (int32_t k = s.inc_ptr[v]; k < s.inc_ptr[v + 1]; ++k) {
+                const int64_t e = s.inc_tet[k];
+                const int32_t* ids = s.conn.data() + 4 * e;
+                const int a = ids[0] == v ? 0 : ids[1] == v ? 1 : ids[2] == v ? 2 : 3;
+                tet_inc[4 * e + a] = k;
+            }
+        upload(h->d_tet_inc, tet_inc.data(), tet_inc.size() * sizeof(int32_t));
+    }
+    h->d_fp_elem.ensure(12 * T * sizeof(double));
+    return h->d_fp_elem.as<double>();
+}
+} // namespace
+} // namespace lpg
+
 lpg_status lpg_tet_energy_force_fp_device(lpg_tet_energy* h, const double* x_dev, const double* theta_dev,
                                           const double* fp_dev, double* out_dev, void* stream) {
     return lpg::derivative_call(h, x_dev, theta_dev, stream, [&](const lpg::DeviceMesh& m, unsigned* st, hipStream_t s) {
-        h->d_fp_elem.ensure(12 * static_cast<size_t>(m.num_tets) * sizeof(double));
-        lpg::launch_force_fp(m, x_dev, theta_dev, fp_dev, nullptr, nullptr, h->d_fp_elem.as<double>(), out_dev, st, s);
+        double* scratch = lpg::fp_scratch(h);
+        lpg::launch_force_fp(m, x_dev, theta_dev, fp_dev, nullptr, nullptr, h->d_tet_inc.as<int32_t>(), scratch,
+                             out_dev, st, s);
     });
 }
 
@@ -586,7 +611,7 @@ lpg_status lpg_tet_energy_force_vjp_fp_device(lpg_tet_energy* h, const double* x
     return lpg::derivative_call(h, x_dev, theta_dev, stream, [&](const lpg::DeviceMesh& m, unsigned* st, hipStream_t s) {
         if (!w_dev && m.num_tets > 0)
             lpg::fail(lpg::kInvalid, "force_vjp_fp: w must have num_dofs entries");
-        lpg::launch_force_fp(m, x_dev, theta_dev, fp_dev, w_dev, nullptr, nullptr, out_dev, st, s);
+        lpg::launch_force_fp(m, x_dev, theta_dev, fp_dev, w_dev, nullptr, nullptr, nullptr, out_dev, st, s);
     });
 }
 
@@ -596,9 +621,9 @@ lpg_status lpg_tet_energy_force_jvp_fp_device(lpg_tet_energy* h, const double* x
     return lpg::derivative_call(h, x_dev, theta_dev, stream, [&](const lpg::DeviceMesh& m, unsigned* st, hipStream_t s) {
         if (!dfp_dev && m.num_tets > 0)
             lpg::fail(lpg::kInvalid, "force_jvp_fp: dFp must have num_tets x 9 entries");
-        h->d_fp_elem.ensure(12 * static_cast<size_t>(m.num_tets) * sizeof(double));
-        lpg::launch_force_fp(m, x_dev, theta_dev, fp_dev, nullptr, dfp_dev, h->d_fp_elem.as<double>(), out_dev, st,
-                             s);
+        double* scratch = lpg::fp_scratch(h);
+        lpg::launch_force_fp(m, x_dev, theta_dev, fp_dev, nullptr, dfp_dev, h->d_tet_inc.as<int32_t>(), scratch,
+                             out_dev, st, s);
     });
 }
 

@@ -97,7 +97,8 @@ struct lpg_tet_energy {
     lpg::DeviceBuffer d_seg_tbeg;    // seg_tbeg on the device (per-mesh feasible steps)
     lpg::DeviceBuffer d_seg_all;     // {0, T}: the whole handle as one segment
     lpg::DeviceBuffer d_fp;          // plastic parts (MeshSetup::fp) when some Fp differs from the identity
-    lpg::DeviceBuffer d_fp_elem;     // element values of the plastic force response (12 doubles per tet)
+    lpg::DeviceBuffer d_fp_elem;     // per-incidence values of the plastic force response (3 doubles each)
+    lpg::DeviceBuffer d_tet_inc;     // incidence index of (tet, slot), 4 per tet (built on first use)
     lpg::TilePlan plan;
     int plan_version = 0;
     lpg::GraphCache graph;

@@ -179,9 +179,10 @@ void launch_force_vjp_theta(const DeviceMesh& m, const double* x, const double* 
                             double* out, unsigned* status, hipStream_t stream);
 // Force response to the plastic part at the per-call Fp (9 per tet, column-major; null: the handle's snapshot):
 // w given: out = (df/dFp)^T w (9 per tet); else out = f (dfp null) or (df/dFp) dfp (num_dofs), through `scratch`
-// (12 doubles per tet).
+// (3 doubles per incidence), written through `tet_inc` (the incidence index of each (tet, slot), 4 per tet).
 void launch_force_fp(const DeviceMesh& m, const double* x, const double* theta, const double* fp, const double* w,
-                     const double* dfp, double* scratch, double* out, unsigned* status, hipStream_t stream);
+                     const double* dfp, const int32_t* tet_inc, double* scratch, double* out, unsigned* status,
+                     hipStream_t stream);
 void launch_force_jvp_theta(const DeviceMesh& m, const double* x, const double* theta, const double* dtheta,
                             double* out, unsigned* status, hipStream_t stream);
 void launch_hvp(const DeviceMesh& m, const double* x, const double* theta, const double* v, double* out,

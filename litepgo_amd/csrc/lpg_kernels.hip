@@ -1176,7 +1176,8 @@ struct FpParams {
     const double* fp;      // per-call Fp (column-major vec, 9 per tet) or null
     const double* w;       // VJP: force cotangent (3 per vertex)
     const double* dfp;     // element mode: direction (9 per tet), or null for the force itself
-    double* out;           // VJP: 9 per tet; element mode: 12 per tet
+    const int32_t* tet_inc; // element mode: incidence index of each (tet, slot), 4 per tet
+    double* out;           // VJP: 9 per tet; element mode: 3 per incidence
     unsigned* status;
     int num_tets;
 };
@@ -1355,7 +1356,8 @@ __global__ __launch_bounds__(256) void k_fp_vjp(FpParams p) {
         atomicOr(p.status, flags);
 }
 
-// Element contributions (12 per tet, vertex slot-major): the force f_b itself (dfp null) or df_b for dfp.
+// Element contributions: the force f_b itself (dfp null) or df_b for dfp, the 3-vector of slot b stored at the
+// incidence of (tet, b), so that each vertex's contributions are contiguous.
 __global__ __launch_bounds__(256) void k_fp_elem(FpParams p) {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= p.num_tets)
@@ -1409,17 +1411,19 @@ __global__ __launch_bounds__(256) void k_fp_elem(FpParams p) {
                 D[3 * i + j] = fma(tr, s.psi[3 * i + j], AdFe[3 * i + j] - pfd);
             }
     }
-    double* out = p.out + 12 * static_cast<size_t>(e);
+    const int4 kinc = reinterpret_cast<const int4*>(p.tet_inc)[e];
     bool fin = true;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         double gp[3];
         plastic_gradient(s, b, gp);
+        const int k = b == 0 ? kinc.x : b == 1 ? kinc.y : b == 2 ? kinc.z : kinc.w;
+        double* out = p.out + 3 * static_cast<size_t>(k);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const double v = s.wp * dot3(D[3 * i], D[3 * i + 1], D[3 * i + 2], gp[0], gp[1], gp[2]);
             fin = fin && isfinite(v);
-            out[3 * b + i] = v;
+            out[i] = v;
         }
     }
     if (!fin)
@@ -1428,17 +1432,14 @@ __global__ __launch_bounds__(256) void k_fp_elem(FpParams p) {
         atomicOr(p.status, flags);
 }
 
-// out_v = sum over the incidences of v of the element values of its slot (12 per tet).  Eight lanes share a vertex
-// and stride over its list (the record holds the slot, so the connectivity is not read); their partial sums are
-// combined by a fixed xor tree, so the result does not depend on timing.
+// out_v = sum of the 3-vectors of v's incidences (contiguous, 3 doubles each).  Eight lanes share a vertex and
+// stride over its range; their partial sums are combined by a fixed xor tree, so the result does not depend on
+// timing.
 constexpr int kElemGatherLanes = 8;
 
-template <bool WIDE>
-__global__ __launch_bounds__(256) void k_elem_gather(const int32_t* __restrict__ inc_ptr, const void* inc_,
+__global__ __launch_bounds__(256) void k_elem_gather(const int32_t* __restrict__ inc_ptr,
                                                      const double* __restrict__ elem, double* __restrict__ out,
                                                      int num_vertices) {
-    using Inc = typename IncT<WIDE>::type;
-    const Inc* __restrict__ inc = static_cast<const Inc*>(inc_);
     const int gid = blockIdx.x * 256 + threadIdx.x;
     const int v = gid / kElemGatherLanes;
     const int sub = gid % kElemGatherLanes;
@@ -1448,8 +1449,7 @@ __global__ __launch_bounds__(256) void k_elem_gather(const int32_t* __restrict__
     double y0 = 0.0, y1 = 0.0, y2 = 0.0;
     const int ke = inc_ptr[v + 1];
     for (int k = inc_ptr[v] + sub; k < ke; k += kElemGatherLanes) {
-        const Inc r = inc[k];
-        const double* src = elem + 12 * static_cast<size_t>(r.tet) + 3 * IncT<WIDE>::slot(r);
+        const double* src = elem + 3 * static_cast<size_t>(k);
         y0 += src[0];
         y1 += src[1];
         y2 += src[2];
@@ -2052,8 +2052,9 @@ void launch_force_jvp_theta(const DeviceMesh& m, const double* x, const double* 
 }
 
 void launch_force_fp(const DeviceMesh& m, const double* x, const double* theta, const double* fp, const double* w,
-                     const double* dfp, double* scratch, double* out, unsigned* status, hipStream_t stream) {
-    FpParams p{m.Xr, m.conn, m.theta, x, theta, m.fp, fp, w, dfp, w ? out : scratch, status, m.num_tets};
+                     const double* dfp, const int32_t* tet_inc, double* scratch, double* out, unsigned* status,
+                     hipStream_t stream) {
+    FpParams p{m.Xr, m.conn, m.theta, x, theta, m.fp, fp, w, dfp, tet_inc, w ? out : scratch, status, m.num_tets};
     const dim3 grid((m.num_tets + 255) / 256), block(256);
     if (w) {
         if (m.num_tets > 0) {
@@ -2071,12 +2072,7 @@ void launch_force_fp(const DeviceMesh& m, const double* x, const double* theta, 
     // Vertices with no incidences (every vertex when there are no tets) get zeros.
     const long long threads = static_cast<long long>(m.num_vertices) * kElemGatherLanes;
     const dim3 ggrid(static_cast<unsigned>((threads + 255) / 256));
-    if (m.wide)
-        hipLaunchKernelGGL(k_elem_gather<true>, ggrid, dim3(256), 0, stream, m.inc_ptr, m.inc, scratch, out,
-                           m.num_vertices);
-    else
-        hipLaunchKernelGGL(k_elem_gather<false>, ggrid, dim3(256), 0, stream, m.inc_ptr, m.inc, scratch, out,
-                           m.num_vertices);
+    hipLaunchKernelGGL(k_elem_gather, ggrid, dim3(256), 0, stream, m.inc_ptr, scratch, out, m.num_vertices);
     check_launch("k_elem_gather");
 }
 
