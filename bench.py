@@ -26,6 +26,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -50,21 +52,59 @@ def read_summary_value(path, key):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(w, nthreads, repeats):
-    """The oracle (oracle/liblpg_oracle.so) on the full 64^3 workload, E+g+H(Projected), median of `repeats`."""
+def cpu_model():
+    """The host CPU's model name from /proc/cpuinfo (or platform.processor())."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or "unknown"
+
+
+def _time_oracle(w, tets, nthreads, repeats):
+    """Median wall time of `repeats` oracle evaluations E+g+H(Projected) of the workload restricted to `tets`, after
+    one untimed warm-up evaluation."""
     from oracle import oracle
 
-    oracle.build()
-    ref = oracle.OracleEnergy(w.num_vertices, w.tets, w.rest, w.theta, nthreads=nthreads)
+    ref = oracle.OracleEnergy(w.num_vertices, w.tets[tets], w.rest, w.theta[tets], nthreads=nthreads)
+    ref.eval(w.u, True, "projected", nthreads=nthreads)
     times = []
     for _ in range(repeats):
         t0 = time.perf_counter()
         ref.eval(w.u, True, "projected", nthreads=nthreads)
         times.append(time.perf_counter() - t0)
-    med = statistics.median(times)
-    return {"value": w.tets.shape[0] / med, "unit": "tets/s", "cores": nthreads, "kind": "port",
-            "sample": f"oracle E+g+H(Projected)+CSR on the full 64^3 workload (1,572,864 tets), median of {repeats} "
-                      f"evals, {nthreads} threads ({med:.2f} s per eval)"}
+    return statistics.median(times)
+
+
+def cpu_baseline(w, nthreads, repeats, single_layers=2, n=64):
+    """The oracle (oracle/liblpg_oracle.so, the C++ restatement of the reference's dense element route, 12x12
+    eigen-clamp and column-owner assembly) timed on this host's cores, after a warm-up, median of `repeats`:
+    at `nthreads` threads on the full 64^3 workload, and at 1 thread on a bounded sample of it (the tets of the
+    first `single_layers` cell layers along x, same deformation)."""
+    from oracle import oracle
+
+    oracle.build()
+    T = w.tets.shape[0]
+    med = _time_oracle(w, np.arange(T), nthreads, repeats)
+    T1 = single_layers * n * n * 6  # cells run x-slowest, six tets each
+    med1 = _time_oracle(w, np.arange(T1), 1, repeats)
+    affinity = len(os.sched_getaffinity(0))
+    return {"value": T / med, "unit": "tets/s", "cores": nthreads, "kind": "port",
+            "sample": f"oracle E+g+H(Projected)+CSR on the full 64^3 workload (1,572,864 tets), {nthreads} threads, "
+                      f"median of {repeats} evals after a warm-up ({med:.2f} s per eval)",
+            "single_thread": {"value": T1 / med1, "unit": "tets/s", "cores": 1,
+                              "sample": f"the first {single_layers} x-layers of the same workload ({T1} tets), "
+                                        f"median of {repeats} after a warm-up ({med1:.3f} s per eval)"},
+            "host": {"cpu_model": cpu_model(), "nproc": os.cpu_count(), "affinity_cores": affinity,
+                     "threads_used": nthreads,
+                     "note": "threads capped at 16, the host-CPU share of one GPU on this machine"},
+            "published_reference": {"value": 0.834e6, "unit": "tets/s",
+                                    "what": "reference Exact Hessian (CSC), 10,368 tets, Apple M1 Pro, 1 thread "
+                                            "(docs/public/benchmark-data/torch-material-baseline/timings.csv:24)"}}
 
 
 def time_steps(dev, step, kind, steps, warmup):
@@ -278,7 +318,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-repeats", type=int, default=3)
+    ap.add_argument("--cpu-repeats", type=int, default=9)
     ap.add_argument("--exact-only", action="store_true", help="time only the Exact-Hessian step (profiling aid)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-mesh batched comparison")
     ap.add_argument("--no-differentiation", action="store_true", help="skip the 32^3 forward/backward timing")

@@ -49,7 +49,7 @@ void upload(DeviceBuffer& buf, const void* src, size_t bytes) {
 BusyScope::BusyScope(std::atomic<bool>& f) : flag(f) {
     bool expected = false;
     if (!flag.compare_exchange_strong(expected, true))
-        fail(kBusy, "MeshEnergy: overlapping evaluation of one energy object");
+        fail(kBusy, "MeshEnergy is already in use by another call");
 }
 BusyScope::~BusyScope() {
     flag.store(false);

@@ -1,22 +1,31 @@
-"""VEG volume-mesh files: read and write, for feeding real tetrahedral meshes to the device operator.
+"""VEG volume-mesh files (the Vega FEM text format litepgo reads, src/core/mesh/io/veg.cpp): read and write, so that
+real tetrahedral meshes can be fed to the device operator.
 
-Follows litepgo's reader and writer (src/core/mesh/io/veg.cpp:1-235; Python binding src/binding/mesh/io/veg.cpp):
+The format, as this reader accepts it:
 
-* lines are trimmed; blank lines and lines starting with '#' are skipped; ``*INCLUDE <file>`` splices in another file
-  (relative to the including file), and an include cycle is an error;
-* ``*VERTICES``: a header ``count 3 ...``, then ``count`` lines ``id x y z`` whose ids run consecutively from 0 or 1
-  (the first line decides);
-* ``*ELEMENTS``: after exactly one vertex section, a type line ``TET`` (4 vertices) or ``CUBIC`` (8), a header
-  ``count width ...`` whose width matches the type, then ``count`` lines ``id v_1 .. v_width`` with vertex ids in range;
-* other records (materials, sets, regions) are ignored; commas and tabs separate numbers like spaces;
-* malformed or truncated data raise ``RuntimeError``, as the reference's ``std::runtime_error`` does through its
-  binding.
+* a file is a sequence of records, one per line; blank lines and ``#`` comments carry nothing, and commas or tabs
+  separate numbers like spaces;
+* a line starting with ``*`` opens a section named by its first word; ``*INCLUDE <file>`` is not a section but is
+  replaced by the records of that file (a path relative to the including file); including a file that is already
+  being read is an error (a cycle);
+* ``*VERTICES``: a header ``count 3 ...`` and ``count`` rows ``id x y z``; ids run consecutively from the first
+  row's id, which must be 0 or 1 and sets the base of every index in the file;
+* ``*ELEMENTS``: the element type (``TET``, 4 vertices, or ``CUBIC``, 8; spaces inside the word are ignored), a
+  header ``count width ...`` with the width the type implies, and ``count`` rows ``id v_1 .. v_width``;
+* every other section (materials, sets, regions, ...) is skipped;
+* a file holds exactly one vertex section, and an element section after it.
 
-The writer emits 1-based ids and 17 significant digits, so a round trip reproduces positions bit for bit.
+Anything else raises ``RuntimeError`` (the reference's std::runtime_error as its binding maps it; its tests pin the
+class and the word "cycle": tests/python/test_mesh.py:46-50,146-150).  The writer emits 1-based ids and 17
+significant digits, so writing and reading back reproduces positions bit for bit.
+
+Design: records come from a generator that expands includes depth-first with the chain of open files; a small
+state machine groups them into sections; each known section is then decoded from its own record list.
 """
 from __future__ import annotations
 
 import os
+import re
 from dataclasses import dataclass
 from pathlib import Path
 
@@ -24,7 +33,8 @@ import numpy as np
 
 __all__ = ["VegMesh", "read_veg", "write_veg"]
 
-_WIDTH = {"TET": 4, "CUBIC": 8}
+_ELEMENT_WIDTH = {"TET": 4, "CUBIC": 8}
+_SPLIT = re.compile(r"[,\s]+")
 
 
 @dataclass
@@ -43,127 +53,159 @@ class VegMesh:
         return int(self.positions.shape[0])
 
 
-def _lines(path: Path, chain: tuple = ()) -> list[str]:
-    canonical = Path(os.path.realpath(path))
-    if canonical in chain:
-        raise RuntimeError(f"veg *INCLUDE cycle through {path}")
-    chain = chain + (canonical,)
+@dataclass
+class _Record:
+    text: str
+    origin: str  # "file:line", for messages
+
+
+def _records(path: Path, open_chain: tuple):
+    """Yields the file's records in order with *INCLUDE expanded; open_chain holds the files being read."""
+    key = os.path.realpath(path)
+    if key in open_chain:
+        names = " -> ".join(os.path.basename(p) for p in open_chain + (key,))
+        raise RuntimeError(f"VEG include cycle: {names}")
     try:
         text = Path(path).read_text()
     except OSError as exc:
-        raise RuntimeError(f"Failed to open veg file: {path}") from exc
-    out = []
-    for raw in text.splitlines():
+        raise RuntimeError(f"VEG: cannot open {path}") from exc
+    chain = open_chain + (key,)
+    for number, raw in enumerate(text.splitlines(), start=1):
         line = raw.strip()
         if not line or line.startswith("#"):
             continue
-        if line.startswith("*INCLUDE "):
-            out.extend(_lines(Path(path).parent / line[len("*INCLUDE "):].strip(), chain))
-            continue
-        out.append(line)
-    return out
+        head = line.split(None, 1)
+        if head[0].upper() == "*INCLUDE":
+            if len(head) < 2:
+                raise RuntimeError(f"VEG: *INCLUDE without a file name at {path}:{number}")
+            yield from _records(Path(path).parent / head[1].strip(), chain)
+        else:
+            yield _Record(line, f"{path}:{number}")
 
 
-def _numbers(line: str, kind, path: Path) -> list:
-    fields = line.replace(",", " ").replace("\t", " ").split()
+def _sections(records):
+    """Groups records into (NAME, header record, body records) in file order; records before any section are an
+    error."""
+    current = None
+    for rec in records:
+        if rec.text.startswith("*"):
+            if current is not None:
+                yield current
+            current = (rec.text[1:].split(None, 1)[0].upper() if len(rec.text) > 1 else "", rec, [])
+        elif current is None:
+            raise RuntimeError(f"VEG: data outside any section at {rec.origin}")
+        else:
+            current[2].append(rec)
+    if current is not None:
+        yield current
+
+
+def _fields(rec: _Record, parse, what: str) -> list:
     try:
-        return [kind(f) for f in fields]
+        return [parse(tok) for tok in _SPLIT.split(rec.text) if tok]
     except ValueError as exc:
-        raise RuntimeError(f"Malformed VEG numeric record in {path}: {line}") from exc
+        raise RuntimeError(f"VEG: non-numeric {what} at {rec.origin}: {rec.text!r}") from exc
+
+
+def _counted_rows(body, header_index: int, header_what: str, check_header):
+    """(header numbers, the rows the header's count announces) of a section body; check_header(header, record)
+    validates the header before its rows are taken."""
+    if len(body) <= header_index:
+        raise RuntimeError(f"VEG: {header_what} header missing")
+    header = _fields(body[header_index], int, f"{header_what} header")
+    if not header or header[0] < 0:
+        raise RuntimeError(f"VEG: bad {header_what} header at {body[header_index].origin}")
+    check_header(header, body[header_index])
+    rows = body[header_index + 1:header_index + 1 + header[0]]
+    if len(rows) < header[0]:
+        raise RuntimeError(f"VEG: {header_what} section ends after {len(rows)} of {header[0]} rows")
+    return header, rows
+
+
+def _decode_vertices(body):
+    def check(header, rec):
+        if len(header) < 2 or header[1] != 3:
+            raise RuntimeError(f"VEG: vertex header must give 3 coordinates per vertex ({rec.origin})")
+
+    header, rows = _counted_rows(body, 0, "vertex", check)
+    positions = np.empty((len(rows), 3), dtype=np.float64)
+    base = None
+    for k, rec in enumerate(rows):
+        vals = _fields(rec, float, "vertex row")
+        if len(vals) != 4 or not vals[0].is_integer():
+            raise RuntimeError(f"VEG: a vertex row is 'id x y z' ({rec.origin}: {rec.text!r})")
+        if base is None:
+            base = int(vals[0])
+            if base not in (0, 1):
+                raise RuntimeError(f"VEG: vertex ids must start at 0 or 1 ({rec.origin})")
+        if int(vals[0]) != base + k:
+            raise RuntimeError(f"VEG: vertex ids must be consecutive ({rec.origin})")
+        positions[k] = vals[1:]
+    if not np.isfinite(positions).all():
+        raise RuntimeError("VEG: non-finite vertex coordinate")
+    return positions, (1 if base is None else base)
+
+
+def _decode_elements(body, num_vertices: int, base: int):
+    if not body:
+        raise RuntimeError("VEG: element type missing")
+    kind = "".join(body[0].text.split()).upper()
+    if kind not in _ELEMENT_WIDTH:
+        raise RuntimeError(f"VEG: element type {kind!r} is not supported (TET or CUBIC) ({body[0].origin})")
+    width = _ELEMENT_WIDTH[kind]
+
+    def check(header, rec):
+        if len(header) < 2 or header[1] != width:
+            raise RuntimeError(f"VEG: {kind} elements have {width} vertices, the header says "
+                               f"{header[1] if len(header) > 1 else '?'} ({rec.origin})")
+
+    header, rows = _counted_rows(body, 1, "element", check)
+    cells = np.empty((len(rows), width), dtype=np.int64)
+    for k, rec in enumerate(rows):
+        vals = _fields(rec, int, "element row")
+        if len(vals) != width + 1:
+            raise RuntimeError(f"VEG: an element row is 'id' and {width} vertex ids ({rec.origin}: {rec.text!r})")
+        cells[k] = vals[1:]
+    cells -= base
+    if cells.size and (cells.min() < 0 or cells.max() >= num_vertices):
+        raise RuntimeError(f"VEG: element vertex id outside 0..{num_vertices - 1} (after the file's base {base})")
+    return cells.astype(np.int32)
 
 
 def read_veg(path) -> VegMesh:
-    path = Path(path)
-    lines = _lines(path)
-    vertices: list | None = None
-    cells: list | None = None
-    width = 0
-    base = 1
-    i = 0
-    while i < len(lines):
-        line = lines[i]
-        if line.startswith("*VERTICES"):
+    """Reads a VEG file (tet or hex elements) into a VegMesh with 0-based cells."""
+    vertices = elements = None
+    for name, rec, body in _sections(_records(Path(path), ())):
+        if name == "VERTICES":
             if vertices is not None:
-                raise RuntimeError(f"Duplicate *VERTICES section in {path}")
-            i += 1
-            if i >= len(lines):
-                raise RuntimeError(f"Missing *VERTICES header in {path}")
-            header = _numbers(lines[i], int, path)
-            i += 1
-            if len(header) < 2 or header[0] < 0 or header[1] != 3:
-                raise RuntimeError(f"Invalid 3D vertex header in {path}")
-            count = header[0]
-            if count > len(lines) - i:
-                raise RuntimeError(f"Truncated vertex data in {path}")
-            vertices = []
-            for row in range(count):
-                values = _numbers(lines[i], float, path)
-                if len(values) < 4:
-                    raise RuntimeError(f"Malformed vertex line in {path}: {lines[i]}")
-                if row == 0 and values[0] == 0:
-                    base = 0
-                if values[0] != row + base:
-                    raise RuntimeError(f"Vertex IDs must be consecutive in {path}")
-                vertices.append(values[1:4])
-                i += 1
-            continue
-        if line.startswith("*ELEMENTS"):
-            if vertices is None or width:
-                raise RuntimeError(f"Expected one *VERTICES section followed by one *ELEMENTS section in {path}")
-            i += 1
-            if i >= len(lines):
-                raise RuntimeError(f"Missing element type in {path}")
-            kind = "".join(lines[i].split())
-            i += 1
-            if kind not in _WIDTH:
-                raise RuntimeError(f"Unsupported element type in {path}: {kind}")
-            width = _WIDTH[kind]
-            if i >= len(lines):
-                raise RuntimeError(f"Missing element count in {path}")
-            header = _numbers(lines[i], int, path)
-            i += 1
-            if not header or header[0] < 0:
-                raise RuntimeError(f"Invalid element count in {path}")
-            count = header[0]
-            if count > len(lines) - i:
-                raise RuntimeError(f"Truncated element data in {path}")
-            if len(header) < 2 or header[1] != width:
-                raise RuntimeError(f"Element width does not match its type in {path}")
-            cells = []
-            for _ in range(count):
-                values = _numbers(lines[i], int, path)
-                if len(values) < width + 1:
-                    raise RuntimeError(f"Malformed element line in {path}: {lines[i]}")
-                ids = values[1:width + 1]
-                if any(v < base or v - base >= len(vertices) for v in ids):
-                    raise RuntimeError(f"Element vertex index out of range in {path}")
-                cells.append([v - base for v in ids])
-                i += 1
-            continue
-        i += 1  # legacy material, set, region and other records
-    if not width:
-        raise RuntimeError(f"No *ELEMENTS section found in {path}")
-    positions = np.asarray(vertices, dtype=np.float64).reshape(-1, 3)
-    if not np.all(np.isfinite(positions)):
-        raise RuntimeError(f"Non-finite vertex position in {path}")
-    return VegMesh(positions, np.asarray(cells, dtype=np.int32).reshape(-1, width))
+                raise RuntimeError(f"VEG: a second *VERTICES section at {rec.origin}")
+            vertices = _decode_vertices(body)
+        elif name == "ELEMENTS":
+            if vertices is None or elements is not None:
+                raise RuntimeError(f"VEG: *ELEMENTS must follow the one *VERTICES section ({rec.origin})")
+            elements = _decode_elements(body, vertices[0].shape[0], vertices[1])
+    if vertices is None or elements is None:
+        raise RuntimeError(f"VEG: {path} has no {'*VERTICES' if vertices is None else '*ELEMENTS'} section")
+    return VegMesh(vertices[0], elements)
 
 
 def write_veg(path, mesh: VegMesh) -> None:
+    """Writes positions and cells as a VEG file (1-based ids, 17 significant digits)."""
     positions = np.asarray(mesh.positions, dtype=np.float64).reshape(-1, 3)
-    cells = np.asarray(mesh.cells).reshape(len(mesh.cells), -1)
-    width = cells.shape[1]
-    if width not in (4, 8):
+    cells = np.asarray(mesh.cells, dtype=np.int64)
+    if cells.ndim != 2 or cells.shape[1] not in (4, 8):
         raise ValueError("write_veg: cells must have 4 (TET) or 8 (CUBIC) vertices")
-    if cells.size and (cells.min() < 0 or cells.max() >= len(positions)):
+    if cells.size and (cells.min() < 0 or cells.max() >= positions.shape[0]):
         raise ValueError("write_veg: cell vertex index out of range")
-    if not np.all(np.isfinite(positions)):
+    if not np.isfinite(positions).all():
         raise ValueError("write_veg: non-finite vertex position")
-    out = [f"*VERTICES\n{len(positions)} 3 0 0\n"]
-    out.extend(f"{v + 1} {p[0]:.17g} {p[1]:.17g} {p[2]:.17g}\n" for v, p in enumerate(positions))
-    out.append(f"*ELEMENTS\n{'TET' if width == 4 else 'CUBIC'}\n{len(cells)} {width} 0\n")
-    out.extend(f"{e + 1} " + " ".join(str(int(v) + 1) for v in c) + "\n" for e, c in enumerate(cells))
+    kind = "TET" if cells.shape[1] == 4 else "CUBIC"
+    out = [f"# {positions.shape[0]} vertices, {cells.shape[0]} elements", "", "*VERTICES",
+           f"{positions.shape[0]} 3 0 0"]
+    out += [f"{i} {x:.17g} {y:.17g} {z:.17g}" for i, (x, y, z) in enumerate(positions.tolist(), start=1)]
+    out += ["", "*ELEMENTS", kind, f"{cells.shape[0]} {cells.shape[1]} 0"]
+    out += [f"{i} " + " ".join(str(v + 1) for v in row) for i, row in enumerate(cells.tolist(), start=1)]
     try:
-        Path(path).write_text("".join(out))
+        Path(path).write_text("\n".join(out) + "\n")
     except OSError as exc:
-        raise RuntimeError(f"Cannot write VEG: {path}") from exc
+        raise RuntimeError(f"VEG: cannot write {path}") from exc

@@ -72,25 +72,39 @@ def test_include_cycle(tmp_path):
     (tmp_path / "cycle.veg").write_text("*INCLUDE cycle.veg\n")
     with pytest.raises(RuntimeError, match="cycle"):
         read_veg(tmp_path / "cycle.veg")
+    # A longer cycle through a second file, and the same file included twice in sequence (not a cycle).
+    (tmp_path / "a.veg").write_text("*INCLUDE b.veg\n")
+    (tmp_path / "b.veg").write_text("# b\n*INCLUDE a.veg\n")
+    with pytest.raises(RuntimeError, match="cycle: a.veg -> b.veg -> a.veg"):
+        read_veg(tmp_path / "a.veg")
+    (tmp_path / "v.inc").write_text(GOOD_VERTICES)
+    (tmp_path / "e.inc").write_text("*SET s\n1\n")
+    (tmp_path / "twice.veg").write_text("*INCLUDE v.inc\n*INCLUDE e.inc\n*INCLUDE e.inc\n*ELEMENTS\nTET\n1 4 0\n"
+                                        "1 1 2 3 4\n")
+    np.testing.assert_array_equal(read_veg(tmp_path / "twice.veg").cells, [[0, 1, 2, 3]])
 
 
 GOOD_VERTICES = "*VERTICES\n4 3 0 0\n1 0 0 0\n2 1 0 0\n3 0 1 0\n4 0 0 1\n"
 
 
 @pytest.mark.parametrize("text,message", [
-    ("*VERTICES\n4 2 0 0\n", "vertex header"),
-    ("*VERTICES\n5 3 0 0\n1 0 0 0\n", "Truncated vertex"),
+    ("*VERTICES\n4 2 0 0\n", "3 coordinates per vertex"),
+    ("*VERTICES\n5 3 0 0\n1 0 0 0\n", "ends after 1 of 5 rows"),
     ("*VERTICES\n2 3 0 0\n1 0 0 0\n3 1 0 0\n*ELEMENTS\nTET\n0 4 0\n", "consecutive"),
-    ("*VERTICES\n1 3 0 0\n1 0 0\n*ELEMENTS\nTET\n0 4 0\n", "Malformed vertex"),
-    ("*VERTICES\n1 3 0 0\n1 0 x 0\n", "Malformed VEG numeric"),
-    (GOOD_VERTICES + GOOD_VERTICES, "Duplicate"),
-    (GOOD_VERTICES + "*ELEMENTS\nPRISM\n1 6 0\n", "Unsupported element type"),
-    (GOOD_VERTICES + "*ELEMENTS\nTET\n1 8 0\n1 1 2 3 4\n", "width"),
-    (GOOD_VERTICES + "*ELEMENTS\nTET\n2 4 0\n1 1 2 3 4\n", "Truncated element"),
-    (GOOD_VERTICES + "*ELEMENTS\nTET\n1 4 0\n1 1 2 3 5\n", "out of range"),
-    (GOOD_VERTICES + "*ELEMENTS\nTET\n1 4 0\n1 1 2 3\n", "Malformed element"),
-    (GOOD_VERTICES, "No \\*ELEMENTS"),
-    ("*ELEMENTS\nTET\n0 4 0\n", "Expected one"),
+    ("*VERTICES\n1 3 0 0\n7 0 0 0\n*ELEMENTS\nTET\n0 4 0\n", "start at 0 or 1"),
+    ("*VERTICES\n1 3 0 0\n1 0 0\n*ELEMENTS\nTET\n0 4 0\n", "'id x y z'"),
+    ("*VERTICES\n1 3 0 0\n1 0 x 0\n", "non-numeric vertex row"),
+    (GOOD_VERTICES + GOOD_VERTICES, "second \\*VERTICES"),
+    (GOOD_VERTICES + "*ELEMENTS\nPRISM\n1 6 0\n", "'PRISM' is not supported"),
+    (GOOD_VERTICES + "*ELEMENTS\nTET\n1 8 0\n1 1 2 3 4\n", "TET elements have 4 vertices"),
+    (GOOD_VERTICES + "*ELEMENTS\nTET\n2 4 0\n1 1 2 3 4\n", "ends after 1 of 2 rows"),
+    (GOOD_VERTICES + "*ELEMENTS\nTET\n1 4 0\n1 1 2 3 5\n", "outside 0..3"),
+    (GOOD_VERTICES + "*ELEMENTS\nTET\n1 4 0\n1 1 2 3\n", "4 vertex ids"),
+    (GOOD_VERTICES + "*ELEMENTS\n", "element type missing"),
+    (GOOD_VERTICES, "no \\*ELEMENTS"),
+    ("*ELEMENTS\nTET\n0 4 0\n", "must follow"),
+    ("1 2 3\n*VERTICES\n", "outside any section"),
+    ("*INCLUDE\n", "without a file name"),
 ])
 def test_errors(tmp_path, text, message):
     (tmp_path / "bad.veg").write_text(text)
@@ -99,5 +113,5 @@ def test_errors(tmp_path, text, message):
 
 
 def test_missing_file(tmp_path):
-    with pytest.raises(RuntimeError, match="Failed to open"):
+    with pytest.raises(RuntimeError, match="cannot open"):
         read_veg(tmp_path / "absent.veg")
