@@ -60,7 +60,8 @@ def test_overlapping_calls_on_one_handle_fail_busy_and_recover(torch):
     n = dev.num_dofs
     lin = torch.from_numpy(-b.load).cuda()
     x0 = torch.zeros(n, dtype=torch.float64, device="cuda")
-    xs = torch.from_numpy(0.01 * np.random.default_rng(0).normal(size=n) * (~b.fixed)).cuda()
+    h = 1.0 / 24
+    xs = torch.from_numpy(np.random.default_rng(0).uniform(-0.05 * h, 0.05 * h, size=n) * (~b.fixed)).cuda()
     E0, g0, H0 = dev.eval_device(xs, True, "projected")
     g0, H0 = g0.clone(), H0.clone()
     opts = solver.NewtonOptions(gradient_tolerance=1e-9)

@@ -169,11 +169,16 @@ def test_beam48_transpose_identity_and_finite_differences(torch, beam48):
     assert abs(lhs - rhs) <= 1e-8 * abs(lhs)
     # Central differences of re-solved equilibria along two theta directions, for L = |x - target|^2.
     x = res.x.cpu().numpy()
-    target = x * (1 + 0.1 * rng.normal(size=n))
+    target = 0.9 * x  # a coherent residual x - target, so the first-order change dominates the FD error
     gL = solver.equilibrium_vjp(dev, b.fixed, res.x, torch.from_numpy(2 * (x - target)).cuda(), opts).cpu().numpy()
     tight = solver.NewtonOptions(gradient_tolerance=1e-10)
-    for d in range(2):
-        dth = rng.normal(size=b.theta.shape) * b.theta * 1e-3
+    # Smooth, coherent directions (a random-sign field over 663,552 tets would leave a first-order change of order
+    # sqrt(T) against third-order terms of order T h^3, i.e. an FD error far above h^2).
+    centroid = b.rest[b.tets].mean(axis=1)
+    fields = [rng.uniform(0.5, 1.5, size=(b.tets.shape[0], 1)),
+              (1.0 + 0.5 * np.sin(2 * np.pi * centroid[:, 0]) * np.cos(np.pi * centroid[:, 2]))[:, None]]
+    for d, field in enumerate(fields):
+        dth = field * b.theta * 1e-3
         vals = []
         for s in (1.0, -1.0):
             th = torch.from_numpy(b.theta + s * dth).cuda()
@@ -183,8 +188,8 @@ def test_beam48_transpose_identity_and_finite_differences(torch, beam48):
         fd = (vals[0] - vals[1]) / 2.0
         ad = float((gL * dth).sum())
         print(f"direction {d}: FD {fd!r} adjoint {ad!r}", flush=True)
-        # Re-solves stop at |g| <= 1e-10 (a change of L far below the ~1e-3 relative difference); the O(h^2)
-        # truncation of steps of 1e-3 theta is ~1e-6 relative.
+        # Re-solves stop at |g| <= 1e-10 (a change of L far below the difference); the O(h^2) truncation of
+        # coherent steps of 1e-3 theta is ~1e-6 relative.
         assert abs(fd - ad) <= 1e-4 * abs(ad)
 
 
