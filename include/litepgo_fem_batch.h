@@ -44,9 +44,13 @@ lpg_status lpg_tet_batch_max_feasible_step_device(lpg_tet_energy* energy, const 
 lpg_status lpg_tet_energy_set_graph(lpg_tet_energy* energy, int32_t enable);
 
 /* Autotuner: times the assembly on this handle's own data (x_dev, the given kind) for each candidate launch
- * shape (lanes per vertex, waves, vertices per tile, LDS budget) and, for batches, persistent grid size, keeps the
- * fastest, and reports it in chosen[5] = {lanes, waves, vertices per tile, LDS budget, persistent grid} with its
- * mean assembly time.  Results stay within rounding of every other shape. */
+ * shape (waves, vertices per tile, LDS budget; 4 lanes per vertex) and, for batches, persistent grid size, keeps
+ * the fastest, and reports it in chosen[5] = {lanes, waves, vertices per tile, LDS budget, persistent grid} with its
+ * mean assembly time (ms, repeats timed runs each).  The choice is cached per process, keyed by the device, the
+ * Hessian kind and the handle's mesh-size histogram (mesh count, tets per mesh in power-of-two bins, vertex degree
+ * class, plastic parts): a later call on a handle with the same key applies the cached choice without timing
+ * (repeats > 0) or times again and replaces it (repeats < 0, |repeats| runs).  No shape changes a bit of the
+ * results. */
 lpg_status lpg_tet_energy_autotune(lpg_tet_energy* energy, const double* x_dev, int32_t hessian_kind,
                                    int32_t repeats, int32_t* chosen, double* best_ms);
 

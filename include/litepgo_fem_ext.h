@@ -1,6 +1,7 @@
 /* litepgo_fem_ext.h -- entry points of the MI355X library that have no counterpart in the reference.
- * They tune or inspect the device execution and never change results beyond floating-point reassociation
- * (the assembly order stays the element order for every configuration). */
+ * They tune or inspect the device execution and never change a bit of the results: every Hessian value, gradient
+ * entry and energy is summed in an order fixed by the mesh alone (each vertex's incidence list, split over 4 lanes
+ * in a fixed pattern), whatever the launch configuration. */
 #ifndef LITEPGO_FEM_EXT_H
 #define LITEPGO_FEM_EXT_H
 
@@ -10,11 +11,12 @@
 extern "C" {
 #endif
 
-/* Launch shape of the assembly kernel: (lanes per vertex, waves per workgroup) in {(1,1), (2,1), (4,1), (8,1),
- * (4,2), (8,2)} ((4,1) only when a vertex has more than 255 neighbours), vertices per tile
- * (<= 64 * waves / lanes_per_vertex) and the LDS budget, in doubles, for what a tile stages (its Hessian segment,
- * incidence records and neighbour lists; a single vertex above it is assembled in HBM instead).  The budget plus
- * 4 doubles per vertex slot must fit one workgroup's LDS.  Used by the autotuner; the default is (4, 1, 16, 4096). */
+/* Launch shape of the assembly kernel: lanes per vertex (always 4: it fixes the summation order), waves per
+ * workgroup in {1, 2, 4} (1 only when a vertex has more than 255 neighbours), vertices per tile
+ * (<= 16 * waves) and the LDS budget, in doubles, for a tile's Hessian segment (a single vertex above it is
+ * assembled in HBM instead, in the same summation order).  The budget plus 4 doubles per vertex slot must fit one
+ * workgroup's LDS.  Used by the autotuner; the default is (4, 1, 16, 4096).  Other lane counts are rejected with
+ * LPG_INVALID_ARGUMENT. */
 lpg_status lpg_tet_energy_set_launch(lpg_tet_energy* energy, int32_t lanes_per_vertex, int32_t waves,
                                      int32_t vertices_per_tile, int32_t lds_budget_doubles);
 /* cfg4 <- {lanes_per_vertex, waves, vertices_per_tile, lds_budget_doubles}. */

@@ -4,7 +4,9 @@
 
 #include <algorithm>
 #include <limits>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -18,8 +20,8 @@ namespace {
 struct Shape {
     int lanes, waves, vpt, budget;
 };
-constexpr Shape kShapes[] = {{4, 1, 16, 4096}, {5, 1, 12, 4096}, {6, 1, 10, 4096}, {4, 2, 32, 6144},
-                             {6, 2, 20, 6144}, {8, 1, 8, 4096},  {2, 1, 32, 6144}, {4, 1, 8, 4096}};
+constexpr Shape kShapes[] = {{4, 1, 16, 4096}, {4, 1, 12, 4096}, {4, 1, 8, 4096}, {4, 2, 32, 6144},
+                             {4, 4, 64, 12288}};
 // Persistent grid sizes tried for batches (0 = one workgroup per tile).
 constexpr int kGrids[] = {0, 1024, 2048, 4096};
 
@@ -40,6 +42,31 @@ double time_assembly(lpg_tet_energy* h, const double* x, int kind, double* g, do
         (void)lpg::run_eval(h, x, nullptr, 1, kind, g, H, energies, h->own_stream);
     h->timing = timing_was;
     return h->timed_evals ? h->time_ms[1] / h->timed_evals : std::numeric_limits<double>::infinity();
+}
+
+// Autotuner choices cached per process.  Key: device, Hessian kind, mesh count, the histogram of tets per mesh in
+// power-of-two bins, wide records and plastic parts; value: {lanes, waves, vertices per tile, LDS budget, grid} and
+// the timed assembly ms.
+struct TuneChoice {
+    int32_t pick[5];
+    double ms;
+};
+std::mutex g_tune_mutex;
+std::map<std::vector<int64_t>, TuneChoice> g_tune_cache;
+
+std::vector<int64_t> tune_key(const lpg_tet_energy* h, int kind) {
+    std::vector<int64_t> key{h->device, kind, static_cast<int64_t>(h->seg_tbeg.size()) - 1,
+                             h->setup.wide ? 1 : 0, h->setup.fp.empty() ? 0 : 1};
+    std::vector<int64_t> bins(33, 0);
+    for (size_t m = 0; m + 1 < h->seg_tbeg.size(); ++m) {
+        const int64_t t = h->seg_tbeg[m + 1] - h->seg_tbeg[m];
+        int b = 0;
+        while (b < 32 && (int64_t{1} << (b + 1)) <= t)
+            ++b;
+        ++bins[t > 0 ? b + 1 : 0];
+    }
+    key.insert(key.end(), bins.begin(), bins.end());
+    return key;
 }
 
 } // namespace
@@ -159,7 +186,31 @@ lpg_status lpg_tet_energy_autotune(lpg_tet_energy* h, const double* x_dev, int32
         const auto& s = h->setup;
         if (!x_dev && s.num_dofs > 0)
             fail(lpg::kInvalid, "Energy::eval: x must have num_dofs finite entries");
-        repeats = std::max(1, repeats);
+        const bool refresh = repeats < 0;
+        repeats = std::max(1, repeats < 0 ? -repeats : repeats);
+        const std::vector<int64_t> key = tune_key(h, hessian_kind);
+        if (!refresh) {
+            TuneChoice c;
+            bool hit = false;
+            {
+                std::lock_guard<std::mutex> lock(g_tune_mutex);
+                const auto it = g_tune_cache.find(key);
+                if (it != g_tune_cache.end()) {
+                    c = it->second;
+                    hit = true;
+                }
+            }
+            if (hit) {
+                lpg::LaunchConfig lc;
+                lc.lanes_per_vertex = c.pick[0];
+                lc.waves = c.pick[1];
+                lpg::set_plan(h, lc, c.pick[2], c.pick[3], c.pick[4]);
+                std::copy(c.pick, c.pick + 5, chosen);
+                if (best_ms)
+                    *best_ms = c.ms;
+                return;
+            }
+        }
         // Scratch outputs on the device for the timing runs.
         lpg::DeviceBuffer g, H, E;
         g.ensure(std::max<size_t>(1, s.num_dofs) * sizeof(double));
@@ -212,6 +263,11 @@ lpg_status lpg_tet_energy_autotune(lpg_tet_energy* h, const double* x_dev, int32
         std::copy(pick, pick + 5, chosen);
         if (best_ms)
             *best_ms = best;
+        TuneChoice c;
+        std::copy(pick, pick + 5, c.pick);
+        c.ms = best;
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        g_tune_cache[key] = c;
     });
 }
 

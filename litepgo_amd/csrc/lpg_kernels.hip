@@ -7,11 +7,15 @@
 // rest positions (~50 FP64 operations), which saves 80 bytes of HBM traffic per tet.
 //
 // Assembly is vertex-owned: the rows 3v..3v+2 of H form one contiguous segment of 9 |nbr(v)| values, and only
-// the lanes of vertex v write it.  The lanes of a vertex split its incident tets (in element order), rebuild F, J,
-// F^-T and log J in registers, and add the three rows of the tet's 12x12 Hessian that belong to v, block by block,
-// to the segment staged in LDS, one lane after the other.  Each value is therefore summed in element order, as the
-// reference's column-owner assembly does (src/core/dof/energy_assembler.cpp:249-273): no atomics on HBM, bitwise
-// reproducible results for every launch shape, and every H value written to HBM once, coalesced.
+// the lanes of vertex v write it.  Four lanes per vertex split its incident tets (lane s takes incidences s, s + 4,
+// ...), rebuild F, J, F^-T and log J in registers, and add the three rows of the tet's 12x12 Hessian that belong to
+// v, block by block, to the segment staged in LDS.  No atomics on HBM, and every H value is written to HBM once,
+// coalesced.  The order in which each value is summed is fixed by the vertex's incidence list (element order, the
+// reference's dof_contributions_ order of src/core/dof/energy_assembler.cpp:48-54) and the fixed 4-lane split, not
+// by the launch configuration, so results are bitwise identical for every launch shape, tile size and persistent
+// grid, and a mesh evaluated inside a batch gives the bits of its own evaluation (the reference guarantees the same
+// across thread counts and batch sizes: assembly at energy_assembler.cpp:123-170).  The lanes' partial sums are
+// not the reference's plain element-order sums, so values agree with the reference's to rounding.
 //
 // The element Hessian uses the closed form of B^T A B for Tet P1 and neo-Hookean (A from neo_hookean.cpp:43-60):
 //   H_ab = w [ mu (g_a . g_b) I3 + lambda q_a q_b^T - p q_b q_a^T ],  q_b = F^-T g_b,  p = lambda log J - mu,
@@ -25,6 +29,7 @@
 // other elements get the 9x9 eigen-clamp of M, and the assembly reconstructs their blocks from proj(M).
 #include "lpg_internal.hpp"
 #include "lpg_eig9.hpp"
+#include "lpg_math.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -273,15 +278,16 @@ __device__ __forceinline__ void select_row(const double G[4][3], int a, double o
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// Assembly kernel: one workgroup per vertex tile.  The LPV lanes of a vertex split its incidence list (lane s takes
-// incidences s, s + LPV, s + 2 LPV, ...).  A lane evaluates its tet in the vertex order (a, a^1, a^2, a^3), which
-// puts the vertex at local slot 0, so the three rows of the element Hessian that belong to the vertex are the
+// Assembly kernel: one workgroup per vertex tile.  The LPV (= 4) lanes of a vertex split its incidence list (lane s
+// takes incidences s, s + LPV, s + 2 LPV, ...).  A lane evaluates its tet in the vertex order (a, a^1, a^2, a^3),
+// which puts the vertex at local slot 0, so the three rows of the element Hessian that belong to the vertex are the
 // blocks H_0j.  The diagonal block H_00 and the gradient and energy contributions are summed per lane in registers
 // and combined in lane order at the end of the tile; the off-diagonal blocks H_0j (j = 1..3) are added to the
-// vertex's H segment in LDS with ds_add_f64 by all lanes at once.  Lanes of one vertex collide on an address only
-// when their tets share the neighbour at the same rotated slot; the order of such colliding adds within one LDS
-// atomic instruction is fixed by the hardware, so results are reproducible run to run for a given LPV and agree
-// across launch shapes to rounding.  KIND: -1 none, 0 Exact, 1 Projected.
+// vertex's H segment in LDS with ds_add_f64, block by block, by all lanes at once.  Lanes of one vertex collide on an
+// address only when their tets share the neighbour; colliding adds within one LDS instruction are applied in
+// ascending lane order, so every value is summed in (round, block, lane) order whatever the tile size or the waves
+// per workgroup.  Tiles above the LDS budget accumulate in HBM with the lanes of a vertex taking turns per block,
+// the same order.  KIND: -1 none, 0 Exact, 1 Projected.  PLASTIC: the handle has plastic parts Fp.
 // ---------------------------------------------------------------------------------------------------------------
 struct AssembleParams {
     const double* Xr;
@@ -304,21 +310,31 @@ struct AssembleParams {
     const double* fp = nullptr; // plastic parts (10 per tet) or null
 };
 
-// The exact blocks H_0j (j = 1..3) and the 6 distinct entries of the symmetric H_00 of one incidence (KIND >= 0),
-// its gradient block and, when the vertex is the tet's slot 0 (a == 0), the tet's energy; t holds the tet in
-// rotated order.  With
-// Ds = [x1-x0 | x2-x0 | x3-x0] (deformed edges) and F = Ds Dm^-1, the vectors q_j = F^-T g_j are the rows of Ds^-1
-// (j = 1..3) and q_0 = -(q_1 + q_2 + q_3); J = det Ds / det Dm.
-template <int KIND>
-__device__ __forceinline__ void incidence_values(const TetIn& t, int a, double off[3][9], double diag[6],
-                                                 double gc[3], double& ec, unsigned& flags) {
-    // Rest geometry.
+// Rest and current positions of a vertex.
+struct VertexData {
+    double X[3];
+    double u[3];
+};
+
+// The values of one incidence, vertex v at rotated slot 0 of its tet, neighbours at slots 1..3: the gradient block
+// gc = w (mu F g_0 + p q_0), the tet's energy ec when v is the tet's slot 0 (a == 0), the 6 distinct entries of
+// the symmetric diagonal block H_00, and each off-diagonal block H_0j passed to emit(j, block[9]) as soon as it is
+// formed.  With Ds = [x1-x0 | x2-x0 | x3-x0] (deformed edges), Dm the rest edges and F = Ds Dm^-1 (Fp^-1):
+//   g_j (j = 1..3) = rows of Dm^-1 (times Fp^-1),  g_0 = -(g_1 + g_2 + g_3),
+//   q_j = rows of Ds^-1 = F^-T g_j,  q_0 = -(q_1 + q_2 + q_3),  J = det Ds / (det Dm det Fp),  p = lambda log J - mu,
+//   H_0j = w [mu (g_0 . g_j) I + lambda q_0 q_j^T - p q_j q_0^T],  H_00 = w [mu |g_0|^2 I + (lambda - p) q_0 q_0^T],
+//   F g_0 = Ds (g_1 . g_0, g_2 . g_0, g_3 . g_0)^T  (Dm^-1 has rows g_j),
+// (neo_hookean.cpp:33-60 for psi, P and A; B^T A B of dof_derivatives.cpp:22-51 in closed form for Tet P1).
+template <int KIND, bool PLASTIC, class Emit>
+__device__ __forceinline__ void incidence_row(const VertexData& v0, const VertexData (&nb)[3], double lam, double mu,
+                                              const double* __restrict__ fp, bool energy, double diag[6],
+                                              double gc[3], double& ec, unsigned& flags, Emit&& emit) {
     double Dm[9], Cm[9];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-            Dm[3 * r + k] = t.X[k + 1][r] - t.X[0][r];
+        for (int c = 0; c < 3; ++c)
+            Dm[3 * r + c] = nb[c].X[r] - v0.X[r];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -328,38 +344,35 @@ __device__ __forceinline__ void incidence_values(const TetIn& t, int a, double o
         }
     const double det_m = dot3(Dm[0], Dm[1], Dm[2], Cm[0], Cm[1], Cm[2]);
     const double inv_m = recip(det_m);
-    double G[4][3]; // rows of dN/dxi Dm^-1
+    double G[3][3]; // rows g_1..g_3
 #pragma unroll
-    for (int b = 1; b < 4; ++b)
+    for (int b = 0; b < 3; ++b)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            G[b][j] = Cm[3 * j + (b - 1)] * inv_m;
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-        G[0][j] = -(G[1][j] + G[2][j] + G[3][j]);
-    // Plastic part: G <- G Fp^-1, w <- w det Fp, J = det Ds / (det Dm det Fp), F = Fp^-1 + Du G (below).
+            G[b][j] = Cm[3 * j + b] * inv_m;
     double w = det_m * (1.0 / 6.0);
     double inv_mp = inv_m;
-    if (t.fp) {
+    if (PLASTIC) {
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
+        for (int b = 0; b < 3; ++b) {
             const double g0 = G[b][0], g1 = G[b][1], g2 = G[b][2];
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-                G[b][j] = dot3(g0, g1, g2, t.fp[j], t.fp[3 + j], t.fp[6 + j]);
+                G[b][j] = dot3(g0, g1, g2, fp[j], fp[3 + j], fp[6 + j]);
         }
-        w *= t.fp[9];
-        inv_mp = inv_m * recip(t.fp[9]);
+        w *= fp[9];
+        inv_mp = inv_m * recip(fp[9]);
     }
-    // Current geometry.
-    double Du[9], Ds[9], Cs[9];
+    double g0[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        g0[j] = -(G[0][j] + G[1][j] + G[2][j]);
+    double Ds[9], Cs[9];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            Du[3 * r + k] = t.u[k + 1][r] - t.u[0][r];
-            Ds[3 * r + k] = Dm[3 * r + k] + Du[3 * r + k];
-        }
+        for (int c = 0; c < 3; ++c)
+            Ds[3 * r + c] = Dm[3 * r + c] + (nb[c].u[r] - v0.u[r]);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -370,82 +383,77 @@ __device__ __forceinline__ void incidence_values(const TetIn& t, int a, double o
     const double det_s = dot3(Ds[0], Ds[1], Ds[2], Cs[0], Cs[1], Cs[2]);
     const double inv_s = recip(det_s);
     const double J = det_s * inv_mp;
-    const double L = log(J);
+    const double L = log_pos(J);
     if (!(det_m > 0.0 && J > 0.0 && isfinite(J) && isfinite(L)))
         flags |= kStatusDetF;
-    if (!theta_ok(t.lam, t.mu))
+    if (!theta_ok(lam, mu))
         flags |= kStatusBadTheta;
-    double q[4][3];
+    double q[3][3], q0[3];
 #pragma unroll
-    for (int b = 1; b < 4; ++b)
+    for (int b = 0; b < 3; ++b)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            q[b][j] = Cs[3 * j + (b - 1)] * inv_s;
+            q[b][j] = Cs[3 * j + b] * inv_s;
 #pragma unroll
     for (int j = 0; j < 3; ++j)
-        q[0][j] = -(q[1][j] + q[2][j] + q[3][j]);
-    // F = I + Du Dm^-1.
-    double F[9];
-    if (!t.fp) {
+        q0[j] = -(q[0][j] + q[1][j] + q[2][j]);
+    // d_j = g_0 . g_j (j = 1..3): the off-diagonal mu terms and Dm^-1 g_0.
+    double d[3];
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+    for (int b = 0; b < 3; ++b)
+        d[b] = dot3(g0[0], g0[1], g0[2], G[b][0], G[b][1], G[b][2]);
+    const double p_ = fma(lam, L, -mu);
+    const double wmu = w * mu, wlam = w * lam, wp = w * p_;
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
-                F[3 * i + j] = fma(Du[3 * i + 2], G[3][j],
-                                   fma(Du[3 * i + 1], G[2][j], fma(Du[3 * i], G[1][j], i == j ? 1.0 : 0.0)));
-    } else {
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-                F[3 * i + j] = fma(Du[3 * i + 2], G[3][j],
-                                   fma(Du[3 * i + 1], G[2][j], fma(Du[3 * i], G[1][j], t.fp[3 * i + j])));
+    for (int i = 0; i < 3; ++i) {
+        const double Fg0 = dot3(Ds[3 * i], Ds[3 * i + 1], Ds[3 * i + 2], d[0], d[1], d[2]);
+        gc[i] = fma(wmu, Fg0, wp * q0[i]);
     }
-    const double p_ = fma(t.lam, L, -t.mu);
-    // Gradient block w (mu F g_0 + p q_0).
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-        gc[i] = w * fma(t.mu, dot3(F[3 * i], F[3 * i + 1], F[3 * i + 2], G[0][0], G[0][1], G[0][2]), p_ * q[0][i]);
-    if (a == 0) {
+    if (energy) {
+        // |F|^2 with F = Ds [g_1; g_2; g_3]: psi = mu/2 (|F|^2 - 3) - mu L + lambda/2 L^2.
         double sq = 0.0;
 #pragma unroll
-        for (int k = 0; k < 9; ++k)
-            sq = fma(F[k], F[k], sq);
-        ec = w * fma(0.5 * t.lam * L, L, fma(0.5 * t.mu, sq - 3.0, -(t.mu * L)));
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double Fij = dot3(Ds[3 * i], Ds[3 * i + 1], Ds[3 * i + 2], G[0][j], G[1][j], G[2][j]);
+                sq = fma(Fij, Fij, sq);
+            }
+        ec = w * fma(0.5 * lam * L, L, fma(0.5 * mu, sq - 3.0, -(mu * L)));
     }
     if (KIND < 0)
         return;
+    double xa[3], ya[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        xa[i] = wlam * q0[i];
+        ya[i] = wp * q0[i];
+    }
     {
-        // H_0j = w [mu (g_0 . g_j) I + lambda q_0 q_j^T - p q_j q_0^T];  H_00 = w [mu |g_0|^2 I + (lambda - p) q_0 q_0^T]
-        const double sw = w * t.mu;
-        double xa[3], ya[3], ca[3];
-        const double c2 = w * (t.lam - p_);
+        const double c1 = wmu * dot3(g0[0], g0[1], g0[2], g0[0], g0[1], g0[2]);
+        double ca[3];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            xa[i] = (w * t.lam) * q[0][i];
-            ya[i] = (w * p_) * q[0][i];
-            ca[i] = c2 * q[0][i];
-        }
-        const double c1 = sw * dot3(G[0][0], G[0][1], G[0][2], G[0][0], G[0][1], G[0][2]);
-        diag[0] = fma(ca[0], q[0][0], c1);
-        diag[1] = ca[0] * q[0][1];
-        diag[2] = ca[0] * q[0][2];
-        diag[3] = fma(ca[1], q[0][1], c1);
-        diag[4] = ca[1] * q[0][2];
-        diag[5] = fma(ca[2], q[0][2], c1);
+        for (int i = 0; i < 3; ++i)
+            ca[i] = xa[i] - ya[i];
+        diag[0] = fma(ca[0], q0[0], c1);
+        diag[1] = ca[0] * q0[1];
+        diag[2] = ca[0] * q0[2];
+        diag[3] = fma(ca[1], q0[1], c1);
+        diag[4] = ca[1] * q0[2];
+        diag[5] = fma(ca[2], q0[2], c1);
+    }
 #pragma unroll
-        for (int j = 1; j < 4; ++j) {
-            const double s1 = sw * dot3(G[0][0], G[0][1], G[0][2], G[j][0], G[j][1], G[j][2]);
+    for (int j = 0; j < 3; ++j) {
+        const double s1 = wmu * d[j];
+        double blk[9];
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int kk = 0; kk < 3; ++kk) {
-                    double val = fma(xa[i], q[j][kk], -(q[j][i] * ya[kk]));
-                    if (i == kk)
-                        val += s1;
-                    off[j - 1][3 * i + kk] = val;
-                }
-        }
+            for (int k = 0; k < 3; ++k) {
+                const double val = fma(xa[i], q[j][k], -(q[j][i] * ya[k]));
+                blk[3 * i + k] = i == k ? val + s1 : val;
+            }
+        emit(j, blk);
     }
 }
 
@@ -476,8 +484,8 @@ __device__ __forceinline__ void load_nbr_data(const AssembleParams& p, const int
 // One tile: zero the tile's H segment in LDS (GLOBAL_ACC: in HBM, for a segment above the LDS budget), accumulate,
 // write it back coalesced, write the gradient rows and per-vertex energies, OR the status flags.
 // LDS layout in doubles: [gradient: 3 per vertex][energy: 1 per vertex][H segment of the tile].
-// GLOBAL_ACC tiles add with plain read-modify-writes to HBM, so their lanes take turns (incidence order).
-template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GLOBAL_ACC>
+// GLOBAL_ACC tiles add with plain read-modify-writes to HBM, the lanes of a vertex taking turns per block.
+template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GLOBAL_ACC, bool PLASTIC>
 __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile, double* lds) {
     using Inc = typename IncT<WIDE>::type;
     constexpr int kThreads = kWave * WAVES;
@@ -540,73 +548,62 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
         // One round: the incidence's values, the register sums, and the off-diagonal blocks into the segment.
         // Indefinite incidences (Projected) keep the records and projection slots of up to two per lane for the
         // second pass; more set `pend_all`, which rescans the lane's incidences.
+        const VertexData me{{Xv.x, Xv.y, Xv.z}, {uv0, uv1, uv2}};
         const auto round = [&](const Inc& r, const NbrData& d, int slot, bool active) {
-            double off[3][9];
-            int cps[3] = {0, 0, 0};
-            bool add = false;
-            if (active) {
+            if (!active)
+                return;
+            int cps[3];
+            VertexData nbd[3];
 #pragma unroll
-                for (int j = 1; j < 4; ++j)
-                    cps[j - 1] = IncT<WIDE>::cp(r, j);
-                TetIn t;
-                t.X[0][0] = Xv.x;
-                t.X[0][1] = Xv.y;
-                t.X[0][2] = Xv.z;
-                t.u[0][0] = uv0;
-                t.u[0][1] = uv1;
-                t.u[0][2] = uv2;
+            for (int j = 0; j < 3; ++j) {
+                cps[j] = IncT<WIDE>::cp(r, j + 1);
 #pragma unroll
-                for (int j = 1; j < 4; ++j)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        t.X[j][c] = d.X[j - 1][c];
-                        t.u[j][c] = d.u[j - 1][c];
-                    }
-                t.lam = d.lam;
-                t.mu = d.mu;
-                t.fp = p.fp ? p.fp + 10 * static_cast<size_t>(r.tet) : nullptr;
-                double gc[3], ec = 0.0, dg[6];
-                incidence_values<KIND>(t, IncT<WIDE>::slot(r), off, dg, gc, ec, flags);
-                gs0 += gc[0];
-                gs1 += gc[1];
-                gs2 += gc[2];
-                es += ec;
-                // Projected: a listed tet's low-rank term is added by the second pass below; its exact blocks are
-                // added here unless the term replaces them (slot <= -2).
-                add = KIND == 0 || (KIND == 1 && slot >= -1);
-                if (KIND == 1 && slot != -1) {
-                    const int li = slot >= 0 ? slot : -2 - slot;
-                    if (pend_s0 < 0) {
-                        pend_s0 = li;
-                        pend_r0 = r;
-                    } else if (pend_s1 < 0) {
-                        pend_s1 = li;
-                        pend_r1 = r;
-                    } else {
-                        pend_all = true;
-                    }
-                }
-                if (add)
-#pragma unroll
-                    for (int c = 0; c < 6; ++c)
-                        ds[c] += dg[c];
-            }
-            if (KIND >= 0) {
-#pragma unroll
-                for (int s = 0; s < (GLOBAL_ACC ? LPV : 1); ++s) {
-                    if (add && (!GLOBAL_ACC || sub == s)) {
-#pragma unroll
-                        for (int j = 0; j < 3; ++j) {
-                            double* blk = row + 3 * cps[j];
-#pragma unroll
-                            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                                for (int kk = 0; kk < 3; ++kk)
-                                    accum<GLOBAL_ACC>(blk + i * rowlen + kk, off[j][3 * i + kk]);
-                        }
-                    }
+                for (int c = 0; c < 3; ++c) {
+                    nbd[j].X[c] = d.X[j][c];
+                    nbd[j].u[c] = d.u[j][c];
                 }
             }
+            const double* fp = PLASTIC ? p.fp + 10 * static_cast<size_t>(r.tet) : nullptr;
+            // Projected: a listed tet's low-rank term is added by the second pass below; its exact blocks are added
+            // here unless the term replaces them (slot <= -2).
+            const bool add = KIND == 0 || (KIND == 1 && slot >= -1);
+            if (KIND == 1 && slot != -1) {
+                const int li = slot >= 0 ? slot : -2 - slot;
+                if (pend_s0 < 0) {
+                    pend_s0 = li;
+                    pend_r0 = r;
+                } else if (pend_s1 < 0) {
+                    pend_s1 = li;
+                    pend_r1 = r;
+                } else {
+                    pend_all = true;
+                }
+            }
+            double gc[3], ec = 0.0, dg[6];
+            incidence_row<KIND, PLASTIC>(me, nbd, d.lam, d.mu, fp, IncT<WIDE>::slot(r) == 0, dg, gc, ec, flags,
+                                         [&](int j, const double(&blk)[9]) {
+                                             if (KIND < 0 || !add)
+                                                 return;
+                                             double* b = row + 3 * cps[j];
+#pragma unroll
+                                             for (int s = 0; s < (GLOBAL_ACC ? LPV : 1); ++s) {
+                                                 if (!GLOBAL_ACC || sub == s) {
+#pragma unroll
+                                                     for (int i = 0; i < 3; ++i)
+#pragma unroll
+                                                         for (int kk = 0; kk < 3; ++kk)
+                                                             accum<GLOBAL_ACC>(b + i * rowlen + kk, blk[3 * i + kk]);
+                                                 }
+                                             }
+                                         });
+            gs0 += gc[0];
+            gs1 += gc[1];
+            gs2 += gc[2];
+            es += ec;
+            if (KIND >= 0 && add)
+#pragma unroll
+                for (int c = 0; c < 6; ++c)
+                    ds[c] += dg[c];
         };
         if (kb < ke) {
             const int k = kb + sub;
@@ -817,21 +814,21 @@ __device__ __forceinline__ int xcd_slot(int b, int G) {
 // budget would only force the two-round pipeline's buffers to spill.
 constexpr int kAssembleMinWaves = 2;
 
-template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GLOBAL_ACC>
+template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GLOBAL_ACC, bool PLASTIC>
 __global__ __launch_bounds__(kWave* WAVES, kAssembleMinWaves) void k_assemble(AssembleParams p) {
     extern __shared__ double lds[];
     const int slot = xcd_slot(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
     const int tile = p.tile_ids ? p.tile_ids[slot] : slot;
-    assemble_tile<LPV, WAVES, WANT_G, KIND, WIDE, GLOBAL_ACC>(p, tile, lds);
+    assemble_tile<LPV, WAVES, WANT_G, KIND, WIDE, GLOBAL_ACC, PLASTIC>(p, tile, lds);
 }
 
 // Persistent form: a fixed grid walks the tile list with a grid stride (the batched launch).
-template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GLOBAL_ACC>
+template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GLOBAL_ACC, bool PLASTIC>
 __global__ __launch_bounds__(kWave* WAVES, kAssembleMinWaves) void k_assemble_persistent(AssembleParams p, int num_tiles) {
     extern __shared__ double lds[];
     for (int t = blockIdx.x; t < num_tiles; t += gridDim.x) {
         const int tile = p.tile_ids ? p.tile_ids[t] : t;
-        assemble_tile<LPV, WAVES, WANT_G, KIND, WIDE, GLOBAL_ACC>(p, tile, lds);
+        assemble_tile<LPV, WAVES, WANT_G, KIND, WIDE, GLOBAL_ACC, PLASTIC>(p, tile, lds);
         __syncthreads();
     }
 }
@@ -1830,12 +1827,12 @@ template <class Kernel> void allow_large_lds(Kernel kern, int waves) {
         fail(kDevice, std::string("hipFuncSetAttribute: ") + hipGetErrorString(err));
 }
 
-template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GACC>
+template <int LPV, int WAVES, bool WANT_G, int KIND, bool WIDE, bool GACC, bool PLASTIC>
 void launch_assemble_g(const AssembleParams& p, int num_tiles, size_t bytes, int persistent_grid,
                        hipStream_t stream) {
     const dim3 block(kWave * WAVES);
     if (persistent_grid > 0) {
-        auto kern = k_assemble_persistent<LPV, WAVES, WANT_G, KIND, WIDE, GACC>;
+        auto kern = k_assemble_persistent<LPV, WAVES, WANT_G, KIND, WIDE, GACC, PLASTIC>;
         static bool configured = false;
         if (!configured) {
             allow_large_lds(kern, WAVES);
@@ -1843,7 +1840,7 @@ void launch_assemble_g(const AssembleParams& p, int num_tiles, size_t bytes, int
         }
         hipLaunchKernelGGL(kern, dim3(std::min(persistent_grid, num_tiles)), block, bytes, stream, p, num_tiles);
     } else {
-        auto kern = k_assemble<LPV, WAVES, WANT_G, KIND, WIDE, GACC>;
+        auto kern = k_assemble<LPV, WAVES, WANT_G, KIND, WIDE, GACC, PLASTIC>;
         static bool configured = false;
         if (!configured) {
             allow_large_lds(kern, WAVES);
@@ -1860,9 +1857,11 @@ void dispatch_kind(const AssembleParams& p, int want_g, int kind, int num_tiles,
                    int persistent_grid, hipStream_t s) {
 #define LPG_CASE(G, K)                                                                                               \
     if (!!want_g == G && kind == K) {                                                                                 \
-        if (gacc)                                                                                                     \
-            return launch_assemble_g<LPV, WAVES, G, K, WIDE, true>(p, num_tiles, bytes, persistent_grid, s);         \
-        return launch_assemble_g<LPV, WAVES, G, K, WIDE, false>(p, num_tiles, bytes, persistent_grid, s);            \
+        if (p.fp)                                                                                                     \
+            return gacc ? launch_assemble_g<LPV, WAVES, G, K, WIDE, true, true>(p, num_tiles, bytes, persistent_grid, s) \
+                        : launch_assemble_g<LPV, WAVES, G, K, WIDE, false, true>(p, num_tiles, bytes, persistent_grid, s); \
+        return gacc ? launch_assemble_g<LPV, WAVES, G, K, WIDE, true, false>(p, num_tiles, bytes, persistent_grid, s)   \
+                    : launch_assemble_g<LPV, WAVES, G, K, WIDE, false, false>(p, num_tiles, bytes, persistent_grid, s); \
     }
     LPG_CASE(false, -1)
     LPG_CASE(true, -1)
@@ -1876,9 +1875,11 @@ void dispatch_kind(const AssembleParams& p, int want_g, int kind, int num_tiles,
 
 } // namespace
 
-// Compiled launch shapes (lanes per vertex, waves per workgroup).  Meshes with a vertex of more than 255 neighbours
-// (16-bit column positions) use (4, 1) only.
-#define LPG_SHAPES(X) X(1, 1) X(2, 1) X(4, 1) X(5, 1) X(6, 1) X(8, 1) X(4, 2) X(6, 2) X(8, 2)
+// Compiled launch shapes (lanes per vertex, waves per workgroup).  The lanes of a vertex split its incidence list
+// and their sums combine in lane order, so the number of lanes per vertex fixes the summation order of every value:
+// it is 4 for every shape, and the shapes differ only in how many vertices (waves) a workgroup takes, which changes
+// no bit of the results.  Meshes with a vertex of more than 255 neighbours (16-bit column positions) use (4, 1).
+#define LPG_SHAPES(X) X(4, 1) X(4, 2) X(4, 4)
 
 bool launch_shape_supported(int lanes_per_vertex, int waves, bool wide) {
     if (wide)
