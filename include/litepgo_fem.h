@@ -6,7 +6,7 @@
  * Conventions (identical to the reference):
  *   - unknowns are displacements u = x - X, DOF 3*v + c for vertex v and axis c (src/core/dof/vertex_dof_map.hpp:43-50);
  *   - theta is per element, (lambda, mu) (src/core/fem/material/volumetric/neo_hookean.hpp:9);
- *   - the Hessian pattern holds both triangles, sorted unique per column (energy_assembler.cpp:146-154).  It is
+ *   - the Hessian pattern holds both triangles, sorted unique per column (energy_assembler.cpp:38-46).  It is
  *     structurally symmetric, so the CSC arrays (outer, inner) of the reference are bitwise the CSR arrays
  *     (row_ptr, col_idx) exported here, and H values are stored in that order;
  *   - the energy is E = sum_e w_e W(F_e) with w_e = det(Dm_e)/6 (volumetric_element.hpp:23-39).
@@ -16,7 +16,7 @@
  * after any error.
  *
  * Device-pointer entry points take a hipStream_t as void*; NULL means the default stream.  They return after the
- * work is finished (the status is read back), except the *_async ones.
+ * work is finished (the status is read back).
  */
 #ifndef LITEPGO_FEM_H
 #define LITEPGO_FEM_H
@@ -51,7 +51,7 @@ size_t lpg_last_error(char* buf, size_t len);
 
 /* Replaces VolumetricFemEnergy<NeoHookean, TetP1Basis, TetCentroidQuadrature>(map, rest_positions, Fp, theta,
  * batch_size) (src/core/fem/assembly/volumetric_fem_energy.hpp:29-65), its MeshEnergy base
- * (src/core/energy/scalar/mesh_energy.cpp:30-42) and EnergyAssembler (src/core/dof/energy_assembler.cpp:130-176).
+ * (src/core/energy/scalar/mesh_energy.cpp:30-42) and EnergyAssembler (src/core/dof/energy_assembler.cpp:22-67).
  *   tets  : num_tets x 4 vertex ids (host), rest: num_vertices x 3 rest positions (host),
  *   theta : num_tets x 2 (lambda, mu) (host), Fp: num_tets x 9 column-major or NULL for the identity.
  *   batch_size is accepted for interface parity (> 0 required); it does not change results.
@@ -74,7 +74,7 @@ int32_t lpg_tet_energy_nnz(const lpg_tet_energy* energy);
  * row_ptr[num_dofs + 1] and col_idx[nnz] (== the reference's CSC outer/inner). */
 lpg_status lpg_tet_energy_pattern(const lpg_tet_energy* energy, int32_t* row_ptr, int32_t* col_idx);
 
-/* EnergyAssembler::hessian_offsets_ in the reference convention (src/core/dof/energy_assembler.cpp:164-175):
+/* EnergyAssembler::hessian_offsets_ in the reference convention (src/core/dof/energy_assembler.cpp:56-67):
  * offsets[144 e + i + 12 j] = position of (row idx_e[i], col idx_e[j]) in the value array.  Host copy. */
 lpg_status lpg_tet_energy_slot_map(const lpg_tet_energy* energy, int32_t* offsets);
 

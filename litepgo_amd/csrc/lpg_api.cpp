@@ -141,9 +141,9 @@ void set_plan(lpg_tet_energy* h, const LaunchConfig& lc, int vertices_per_tile, 
 
 // Device state is created on the first call that needs the GPU, so construction, the pattern and the slot map
 // work on hosts without one.
-void ensure_device(lpg_tet_energy* h) {
-    // A rejected plastic part fails every evaluation (the reference raises it per evaluated element).
-    if (h->setup.fp_error)
+void ensure_device(lpg_tet_energy* h, bool snapshot_fp) {
+    // A rejected plastic part fails every evaluation that uses it (the reference raises it per evaluated element).
+    if (snapshot_fp && h->setup.fp_error)
         fail(h->setup.fp_error, h->setup.fp_error_msg);
     if (h->device_ready) {
         hip_check(hipSetDevice(h->device), "hipSetDevice");
@@ -219,6 +219,10 @@ void raise_status(unsigned status) {
         fail(kInvalid, "Energy::eval: x must have num_dofs finite entries");
     if (status & kStatusBadTheta)
         fail(kInvalid, "NeoHookean requires finite lambda >= 0 and mu > 0");
+    if (status & kStatusNonFiniteFp)
+        fail(kInvalid, "MultiplicativeKinematics requires finite F and Fp");
+    if (status & kStatusBadDetFp)
+        fail(kDomain, "MultiplicativeKinematics requires finite det(Fp) > 0");
     if (status & kStatusDetF)
         fail(kDomain, "NeoHookean requires finite det(Fe) > 0");
     if (status & kStatusProjection)
@@ -396,7 +400,8 @@ void eval_device(lpg_tet_energy* h, const double* x, const double* theta, int wa
 }
 
 template <class Launch>
-lpg_status derivative_call(lpg_tet_energy* h, const double* x, const double* theta, void* stream, Launch&& launch) {
+lpg_status derivative_call(lpg_tet_energy* h, const double* x, const double* theta, void* stream, Launch&& launch,
+                           bool snapshot_fp = true) {
     return guarded([&] {
         if (!h)
             fail(kInvalid, "null handle");
@@ -405,7 +410,7 @@ lpg_status derivative_call(lpg_tet_energy* h, const double* x, const double* the
             fail(kInvalid, "Energy::eval: x must have num_dofs finite entries");
         if (!theta && !h->setup.theta_valid && h->setup.num_tets > 0)
             fail(kInvalid, "NeoHookean requires finite lambda >= 0 and mu > 0");
-        ensure_device(h);
+        ensure_device(h, snapshot_fp);
         const hipStream_t s = static_cast<hipStream_t>(stream);
         launch_clear(h->d_status(), h->d_counter(), s);
         launch(h->mesh(), h->d_status(), s);
@@ -602,7 +607,7 @@ lpg_status lpg_tet_energy_force_fp_device(lpg_tet_energy* h, const double* x_dev
         double* scratch = lpg::fp_scratch(h);
         lpg::launch_force_fp(m, x_dev, theta_dev, fp_dev, nullptr, nullptr, h->d_tet_inc.as<int32_t>(), scratch,
                              out_dev, st, s);
-    });
+    }, fp_dev == nullptr);
 }
 
 lpg_status lpg_tet_energy_force_vjp_fp_device(lpg_tet_energy* h, const double* x_dev, const double* theta_dev,
@@ -612,7 +617,7 @@ lpg_status lpg_tet_energy_force_vjp_fp_device(lpg_tet_energy* h, const double* x
         if (!w_dev && m.num_tets > 0)
             lpg::fail(lpg::kInvalid, "force_vjp_fp: w must have num_dofs entries");
         lpg::launch_force_fp(m, x_dev, theta_dev, fp_dev, w_dev, nullptr, nullptr, nullptr, out_dev, st, s);
-    });
+    }, fp_dev == nullptr);
 }
 
 lpg_status lpg_tet_energy_force_jvp_fp_device(lpg_tet_energy* h, const double* x_dev, const double* theta_dev,
@@ -624,7 +629,7 @@ lpg_status lpg_tet_energy_force_jvp_fp_device(lpg_tet_energy* h, const double* x
         double* scratch = lpg::fp_scratch(h);
         lpg::launch_force_fp(m, x_dev, theta_dev, fp_dev, nullptr, dfp_dev, h->d_tet_inc.as<int32_t>(), scratch,
                              out_dev, st, s);
-    });
+    }, fp_dev == nullptr);
 }
 
 lpg_status lpg_tet_energy_hvp_device(lpg_tet_energy* h, const double* x_dev, const double* theta_dev,

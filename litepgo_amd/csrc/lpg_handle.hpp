@@ -125,8 +125,9 @@ struct lpg_tet_energy {
 
 namespace lpg {
 
-// Uploads the mesh and creates the stream and read-back buffer on first use.
-void ensure_device(lpg_tet_energy* h);
+// Uploads the mesh and creates the stream and read-back buffer on first use.  A plastic part rejected at
+// construction fails the call unless the call brings its own plastic parts (snapshot_fp false).
+void ensure_device(lpg_tet_energy* h, bool snapshot_fp = true);
 // Launch shapes accepted by lpg_tet_energy_set_launch (and tried by the autotuner).
 bool launch_config_ok(const lpg_tet_energy* h, const LaunchConfig& lc, int vertices_per_tile, int lds_budget);
 // (Re)builds the tiles for a launch configuration.

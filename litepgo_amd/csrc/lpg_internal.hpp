@@ -29,6 +29,8 @@ enum StatusBits : unsigned {
     kStatusNonFiniteOut = 8u,   // non-finite element or assembled outputs       -> domain_error
     kStatusProjection = 16u,    // projection failed to converge / non-finite    -> domain_error
     kStatusNonFiniteStep = 32u, // non-finite deformation-gradient increment     -> domain_error
+    kStatusNonFiniteFp = 64u,   // a per-call plastic part with a non-finite entry -> invalid_argument
+    kStatusBadDetFp = 128u,     // a per-call plastic part with det <= 0 or non-finite -> domain_error
 };
 
 // Device layout (no per-tet geometry is stored; Dm^-1 and w are recomputed from the rest positions):

@@ -709,12 +709,14 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
                     for (int c = 0; c < 6; ++c)
                         ds[c] += pv[c];
                 }
+                // Block by block, the lanes of a vertex taking turns within a block in HBM accumulation: the
+                // (block, lane) order of the LDS adds.
 #pragma unroll
-                for (int s = 0; s < (GLOBAL_ACC ? LPV : 1); ++s) {
-                    if (add && (!GLOBAL_ACC || sub == s)) {
+                for (int j = 0; j < 3; ++j) {
+                    double* blk = row + 3 * cps[j];
 #pragma unroll
-                        for (int j = 0; j < 3; ++j) {
-                            double* blk = row + 3 * cps[j];
+                    for (int s = 0; s < (GLOBAL_ACC ? LPV : 1); ++s) {
+                        if (add && (!GLOBAL_ACC || sub == s)) {
 #pragma unroll
                             for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -1197,11 +1199,14 @@ __device__ __forceinline__ void fp_state(const FpParams& p, int e, int ids[4], F
     double detp = 1.0;
     if (p.fp) {
         double Fp[9]; // row-major from the column-major input
+        bool fp_finite = true;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
+            for (int j = 0; j < 3; ++j) {
                 Fp[3 * i + j] = p.fp[9 * static_cast<size_t>(e) + i + 3 * j];
+                fp_finite = fp_finite && isfinite(Fp[3 * i + j]);
+            }
         double C[9];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -1211,8 +1216,11 @@ __device__ __forceinline__ void fp_state(const FpParams& p, int e, int ids[4], F
                 C[3 * i + j] = fma(Fp[3 * i1 + j1], Fp[3 * i2 + j2], -(Fp[3 * i1 + j2] * Fp[3 * i2 + j1]));
             }
         detp = dot3(Fp[0], Fp[1], Fp[2], C[0], C[1], C[2]);
-        if (!(detp > 0.0) || !isfinite(detp))
-            flags |= kStatusDetF; // MultiplicativeKinematics: det Fp must be positive and finite
+        // MultiplicativeKinematics (multiplicative_kinematics.cpp:9-16): finite Fp, then finite det Fp > 0.
+        if (!fp_finite)
+            flags |= kStatusNonFiniteFp;
+        else if (!(detp > 0.0) || !isfinite(detp))
+            flags |= kStatusBadDetFp;
         const double inv = 1.0 / detp;
 #pragma unroll
         for (int i = 0; i < 3; ++i)

@@ -1,7 +1,14 @@
 // Scalar FP64 helpers shared by the kernels and the host tests.
 #pragma once
 
+#if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
+#else
+// Plain host compilers (the CPU tests build this header with g++).
+#define __host__
+#define __device__
+#define __forceinline__ inline
+#endif
 
 #include <cmath>
 

@@ -1,7 +1,7 @@
 // Host-side construction: validation, the device arrays, the sparsity pattern, vertex incidence lists and tiles.
 //
 // The pattern is built at vertex granularity.  The reference forms every (row, col) pair of every element's 12
-// DOFs and deduplicates them with setFromTriplets (src/core/dof/energy_assembler.cpp:146-154); because a Tet P1
+// DOFs and deduplicates them with setFromTriplets (src/core/dof/energy_assembler.cpp:38-46); because a Tet P1
 // element couples all 3 DOFs of its 4 vertices, that set is exactly {(3v+i, 3w+k) : v, w share a tet}, so each
 // vertex's sorted neighbour list (itself included) expands to its three rows.  Row 3v+i of the CSR matrix is
 // therefore [3w, 3w+1, 3w+2 for w in nbr(v)], and rows 3v, 3v+1, 3v+2 are contiguous in the value array: a vertex
@@ -192,7 +192,7 @@ void build_mesh_setup(MeshSetup& s, int32_t nv, int32_t nt, const int32_t* tets,
         }
     }
 
-    // Incidences, element order per vertex (the reference's dof_contributions_ order, energy_assembler.cpp:156-162).
+    // Incidences, element order per vertex (the reference's dof_contributions_ order, energy_assembler.cpp:48-54).
     s.inc_ptr.assign(static_cast<size_t>(nv) + 1, 0);
     for (int64_t k = 0; k < 4 * static_cast<int64_t>(nt); ++k)
         ++s.inc_ptr[tets[k] + 1];

@@ -226,6 +226,7 @@ class _ForceFp(torch.autograd.Function):
         p = potential
         xd = _device_input(x, "x", p.num_dofs)
         out = p.force_fp_device(xd, _device_input(Fp, "Fp", 9 * p.num_tets), theta=_theta_input(p, theta))
+        ctx.set_materialize_grads(False)  # absent tangents of x and theta arrive as None, not zeros
         ctx.save_for_backward(Fp, x, theta)
         ctx.save_for_forward(Fp, x, theta)
         ctx.potential = p
@@ -235,6 +236,9 @@ class _ForceFp(torch.autograd.Function):
     @once_differentiable
     def backward(ctx, w):
         Fp, x, theta = ctx.saved_tensors
+        if ctx.needs_input_grad[1] or ctx.needs_input_grad[2]:
+            raise RuntimeError("force_fp: only Fp is differentiable; pass x and theta without requires_grad (use "
+                               "energy_and_force for derivatives in x and theta)")
         if not ctx.needs_input_grad[0]:
             return None, None, None, None
         p = ctx.potential
@@ -243,9 +247,11 @@ class _ForceFp(torch.autograd.Function):
         return _out(g, Fp).reshape(Fp.shape), None, None, None
 
     @staticmethod
-    def jvp(ctx, Fp_dot, _x_dot, _theta_dot, _potential):
+    def jvp(ctx, Fp_dot, x_dot, theta_dot, _potential):
         Fp, x, theta = ctx.saved_tensors
         p = ctx.potential
+        if x_dot is not None or theta_dot is not None:
+            raise RuntimeError("force_fp: only Fp is differentiable; pass x and theta without forward tangents")
         if Fp_dot is None:
             return torch.zeros(p.num_dofs, dtype=torch.float64, device=x.device)
         out = p.force_jvp_fp_device(_device_input(x, "x", p.num_dofs),

@@ -12,8 +12,8 @@
 //   * g_e = w B^T vec(P) and H_e = w B^T A B as dense products (dof_derivatives.cpp:22-51),
 //   * the Projected Hessian is the eigenvalue clamp of the symmetrized 12x12 element matrix
 //     (mesh_energy.cpp:16-27; a cyclic Jacobi eigensolver stands in for Eigen's SelfAdjointEigenSolver),
-//   * assembly is the deterministic column-owner scatter in element order (energy_assembler.cpp:231-278),
-//   * the CSC pattern and the slot map are those of energy_assembler.cpp:130-176.
+//   * assembly is the deterministic column-owner scatter in element order (energy_assembler.cpp:123-170),
+//   * the CSC pattern and the slot map are those of energy_assembler.cpp:22-67.
 // Every function below names the reference lines it restates.  Errors map to the reference's exception classes:
 // 1 = std::invalid_argument, 2 = std::domain_error, 3 = std::logic_error.
 #include <algorithm>
@@ -290,11 +290,11 @@ struct Energy {
     std::vector<double> Fp;       // T x 9 column-major
     std::vector<double> theta;    // T x 2
     bool theta_valid = true;
-    // CSC pattern (== CSR of the symmetric pattern), energy_assembler.cpp:146-154.
+    // CSC pattern (== CSR of the symmetric pattern), energy_assembler.cpp:38-46.
     std::vector<int32_t> outer, inner;
-    // hessian_offsets_[e][i + 12 j], energy_assembler.cpp:164-175.
+    // hessian_offsets_[e][i + 12 j], energy_assembler.cpp:56-67.
     std::vector<int32_t> offsets;
-    // dof_contributions_, energy_assembler.cpp:156-162 (CSR over DOFs).
+    // dof_contributions_, energy_assembler.cpp:48-54 (CSR over DOFs).
     std::vector<int64_t> contrib_ptr;
     std::vector<Contribution> contrib;
     std::atomic<bool> busy{false};
@@ -340,7 +340,7 @@ template <class F> void parallel_for(int64_t count, int nthreads, F&& body) {
 
 // Construction: VolumetricFemEnergy ctor (volumetric_fem_energy.hpp:29-65), VolumetricElement3D
 // (volumetric_element.hpp:23-39), deformation_gradient_sampling_response (..._response.hpp:15-45),
-// EnergyAssembler ctor (energy_assembler.cpp:130-176), MeshTopology checks (mesh_topology.hpp:25-43).
+// EnergyAssembler ctor (energy_assembler.cpp:22-67), MeshTopology checks (mesh_topology.hpp:25-43).
 void build(Energy& E, int nv, int nt, const int32_t* cells, const double* rest, const double* theta,
            const double* Fp, int nthreads) {
     if (nv < 0 || nt < 0)
@@ -415,7 +415,7 @@ void build(Energy& E, int nv, int nt, const int32_t* cells, const double* rest, 
         if (!std::isfinite(l) || !std::isfinite(m) || l < 0.0 || m <= 0.0)
             E.theta_valid = false;
     }
-    // dof_contributions_ (energy_assembler.cpp:156-162): per DOF, (element, local) in element order.
+    // dof_contributions_ (energy_assembler.cpp:48-54): per DOF, (element, local) in element order.
     const int n = E.num_dofs;
     E.contrib_ptr.assign(n + 1, 0);
     for (int e = 0; e < nt; ++e)
@@ -431,7 +431,7 @@ void build(Energy& E, int nv, int nt, const int32_t* cells, const double* rest, 
                 E.contrib[fill[dof(E, e, l)]++] = {e, l};
     }
     // Pattern: all (row, col) with row, col in idx(e); sorted unique rows per column (setFromTriplets +
-    // pattern_of, energy_assembler.cpp:146-154).  Built column by column from the contribution lists.
+    // pattern_of, energy_assembler.cpp:38-46).  Built column by column from the contribution lists.
     std::vector<std::vector<int32_t>> rows(n);
     parallel_for(n, nthreads, [&](int64_t col) {
         auto& r = rows[col];
@@ -455,7 +455,7 @@ void build(Energy& E, int nv, int nt, const int32_t* cells, const double* rest, 
         std::copy(rows[c].begin(), rows[c].end(), E.inner.begin() + E.outer[c]);
         std::vector<int32_t>().swap(rows[c]);
     });
-    // hessian_offsets_[e][i + 12 j] = lower_bound(row idx[i] in column idx[j]) (energy_assembler.cpp:164-175).
+    // hessian_offsets_[e][i + 12 j] = lower_bound(row idx[i] in column idx[j]) (energy_assembler.cpp:56-67).
     E.offsets.assign(static_cast<size_t>(nt) * 144, 0);
     parallel_for(nt, nthreads, [&](int64_t e) {
         for (int j = 0; j < 12; ++j) {
@@ -660,7 +660,7 @@ struct BusyScope {
     }
 };
 
-// MeshEnergy::eval_values (mesh_energy.cpp:48-87) + EnergyAssembler::assemble (energy_assembler.cpp:231-278).
+// MeshEnergy::eval_values (mesh_energy.cpp:48-87) + EnergyAssembler::assemble (energy_assembler.cpp:123-170).
 double eval(Energy& E, const double* x, const double* theta_override, bool want_g, int kind, int nthreads, double* g,
             double* Hval) {
     check_x(E, x);

@@ -6,7 +6,7 @@ neo-Hookean path.  It shares no code with the oracle (oracle/lpg_oracle.cpp) or 
     numpy.kron (deformation_gradient_sampling_response.hpp:30);
   * Projected: LAPACK (numpy.linalg.eigh) eigen-clamp of the symmetrized 12x12 (mesh_energy.cpp:16-27);
   * pattern and values: scipy COO -> CSC with duplicate summation, i.e. setFromTriplets
-    (energy_assembler.cpp:146-154); slot map by numpy.searchsorted (energy_assembler.cpp:164-175).
+    (energy_assembler.cpp:38-46); slot map by numpy.searchsorted (energy_assembler.cpp:56-67).
 
 Fixtures (all .npz, loaded with allow_pickle=False):
   two_tet.npz : the 2-tet mesh of tests/cpp/fem/assembly/volumetric_fem_energy_tet_p1_gtest.cpp:26-46, theta=(2,3),

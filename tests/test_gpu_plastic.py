@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 import scipy.sparse as sp
 
+from litepgo_amd._native import DomainError
 from litepgo_amd import VolumetricFemEnergy, kuhn_grid, lame_parameters
 
 pytestmark = pytest.mark.gpu
@@ -158,14 +159,28 @@ def test_force_fp_response_errors_and_unused_vertex(torch, oracle_mod):
     fr, _, jvr = oracle_mod.force_fp_products(tets, X, theta, u, fp, dFp=d)
     assert rel(f, fr) <= TOL and rel(jv, jvr) <= TOL
     assert np.all(f[-3:] == 0.0) and np.all(jv[-3:] == 0.0)
+    # MultiplicativeKinematics' two rejections (multiplicative_kinematics.cpp:9-16): a non-finite Fp is an
+    # invalid_argument, a non-positive det Fp a domain_error, each with the reference's message.
     bad = fp.copy()
-    bad[5] = (-np.eye(3)).ravel()  # det Fp < 0: MultiplicativeKinematics' domain error
-    with pytest.raises(ValueError):
-        dev.force_fp_device(x, torch.from_numpy(bad).cuda())
-    with pytest.raises(ValueError):
-        dev.force_vjp_fp_device(x, x, torch.from_numpy(bad).cuda())
+    bad[5] = (-np.eye(3)).ravel()
+    for call in (lambda b: dev.force_fp_device(x, b), lambda b: dev.force_vjp_fp_device(x, x, b),
+                 lambda b: dev.force_jvp_fp_device(x, torch.from_numpy(d).cuda(), b)):
+        with pytest.raises(DomainError, match="MultiplicativeKinematics requires finite det"):
+            call(torch.from_numpy(bad).cuda())
+    nanfp = fp.copy()
+    nanfp[7, 4] = np.nan
+    with pytest.raises(ValueError, match="MultiplicativeKinematics requires finite F and Fp") as info:
+        dev.force_fp_device(x, torch.from_numpy(nanfp).cuda())
+    assert not isinstance(info.value, DomainError)
     # The handle recovers after the error.
     assert rel(dev.force_fp_device(x, torch.from_numpy(fp).cuda()).cpu().numpy(), fr) <= TOL
+    # A handle whose snapshot Fp was rejected still answers calls that bring their own plastic parts.
+    snap = np.stack([np.eye(3)] * T)
+    snap[2] = -np.eye(3)
+    rejected = VolumetricFemEnergy(len(X), tets, X, theta, Fp=snap)
+    with pytest.raises(DomainError):
+        rejected.force_fp_device(x)
+    assert rel(rejected.force_fp_device(x, torch.from_numpy(fp).cuda()).cpu().numpy(), fr) <= TOL
 
 
 def test_force_fp_fits_a_plastic_scale(torch):

@@ -213,3 +213,29 @@ def test_force_fp_reverse_and_forward_modes(torch):
         tangent = fwAD.unpack_dual(lt.force_fp(pot, fwAD.make_dual(F0, d), x)).tangent
     scale = float(torch.abs(fd).max())
     assert float(torch.abs(tangent - fd).max()) <= 1e-6 * scale
+
+
+def test_force_fp_rejects_gradients_in_x_and_theta(torch):
+    """force_fp differentiates in Fp only: asking autograd for df/dx or df/dtheta through it raises instead of
+    returning zeros."""
+    from litepgo_amd import VolumetricFemEnergy, kuhn_grid, lame_parameters
+    from litepgo_amd import torch as lt
+
+    X, tets = kuhn_grid(2)
+    T = tets.shape[0]
+    lam, mu = lame_parameters(1e5, 0.4)
+    pot = VolumetricFemEnergy(len(X), tets, X, np.tile([lam, mu], (T, 1)))
+    rng = np.random.default_rng(4)
+    Fp = torch.from_numpy(np.tile(np.eye(3).reshape(1, 9), (T, 1))).cuda().requires_grad_(True)
+    x = torch.from_numpy(0.01 * rng.normal(size=X.size)).cuda().requires_grad_(True)
+    f = lt.force_fp(pot, Fp, x)
+    with pytest.raises(RuntimeError, match="only Fp is differentiable"):
+        torch.autograd.grad(f.sum(), (x,))
+    theta = torch.from_numpy(np.tile([lam, mu], (T, 1))).cuda().requires_grad_(True)
+    f = lt.force_fp(pot, Fp, x.detach(), theta)
+    with pytest.raises(RuntimeError, match="only Fp is differentiable"):
+        torch.autograd.grad(f.sum(), (theta,))
+    # Fp alone still works.
+    f = lt.force_fp(pot, Fp, x.detach())
+    (g,) = torch.autograd.grad(f.sum(), (Fp,))
+    assert g.shape == Fp.shape and torch.isfinite(g).all()
