@@ -243,6 +243,8 @@ double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter) {
     std::memcpy(&counter, static_cast<char*>(h->pinned) + 12, 4);
     if (want_counter)
         h->last_indefinite = static_cast<int32_t>(counter);
+    if (status & kStatusProjOverflow)
+        throw ProjectionOverflow(static_cast<int32_t>(counter));
     raise_status(status);
     return value;
 }
@@ -273,10 +275,15 @@ void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int w
         const size_t T = static_cast<size_t>(h->setup.num_tets);
         h->d_proj_slot.ensure(std::max<size_t>(1, T) * sizeof(int32_t));
         h->d_proj_list.ensure(std::max<size_t>(1, T) * sizeof(int32_t));
-        h->d_proj_H.ensure(std::max<size_t>(1, T) * kProjPerTet * sizeof(double));
+        // The low-rank store holds the listed (not certified positive definite) tets only: 1/16 of the tets to
+        // start with, grown to the listed count when an evaluation lists more (run_eval repeats it then).
+        if (h->proj_capacity == 0)
+            h->proj_capacity = static_cast<int32_t>(std::min<size_t>(T, std::max<size_t>(4096, T / 16)));
+        h->d_proj_H.ensure(std::max<size_t>(1, h->proj_capacity) * kProjPerTet * sizeof(double));
         a.proj_slot = h->d_proj_slot.as<int32_t>();
         a.proj_list = h->d_proj_list.as<int32_t>();
         a.proj_H = h->d_proj_H.as<double>();
+        a.proj_capacity = h->proj_capacity;
     }
     if (h->timing)
         hip_check(hipEventRecord(h->ev[0], stream), "hipEventRecord");
@@ -337,14 +344,35 @@ double read_back(lpg_tet_energy* h, bool want_counter) {
     std::memcpy(&counter, static_cast<char*>(h->pinned) + 12, 4);
     if (want_counter)
         h->last_indefinite = static_cast<int32_t>(counter);
+    if (status & kStatusProjOverflow)
+        throw ProjectionOverflow(static_cast<int32_t>(counter));
     raise_status(status);
     return value;
 }
+
+double run_eval_once(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g,
+                     double* H, double* energies, hipStream_t stream);
 
 } // namespace
 
 double run_eval(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g, double* H,
                 double* energies, hipStream_t stream) {
+    try {
+        return run_eval_once(h, x, theta, want_g, kind, g, H, energies, stream);
+    } catch (const ProjectionOverflow& o) {
+        // More tets were listed for projection than the store holds: grow it to the listed count (plus an eighth)
+        // and evaluate again; a captured graph holds the old store, so it is dropped.
+        const int64_t T = h->setup.num_tets;
+        h->proj_capacity = static_cast<int32_t>(std::min<int64_t>(T, int64_t{o.listed} + o.listed / 8 + 1024));
+        h->graph.reset();
+        return run_eval_once(h, x, theta, want_g, kind, g, H, energies, stream);
+    }
+}
+
+namespace {
+
+double run_eval_once(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g,
+                     double* H, double* energies, hipStream_t stream) {
     ensure_device(h);
     GraphCache& gc = h->graph;
     if (gc.enabled && !h->timing) {
@@ -389,8 +417,6 @@ double run_eval(lpg_tet_energy* h, const double* x, const double* theta, int wan
     collect_timing(h);
     return value;
 }
-
-namespace {
 
 void eval_device(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* value,
                  double* g, double* H, hipStream_t stream) {

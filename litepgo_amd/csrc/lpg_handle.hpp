@@ -104,6 +104,7 @@ struct lpg_tet_energy {
     lpg::GraphCache graph;
     std::atomic<bool> busy{false};
     int32_t last_indefinite = 0;
+    int32_t proj_capacity = 0; // list entries d_proj_H holds (grown when an evaluation lists more tets)
     void* pinned = nullptr; // energy + status + counter read-back
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -153,6 +154,12 @@ void collect_timing(lpg_tet_energy* h);
 double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter);
 // The status mapping alone (status word as read back).
 void raise_status(unsigned status);
+// Raised by finish()/read-back when the projection store was too small; run_eval grows it and evaluates again.
+struct ProjectionOverflow : std::runtime_error {
+    int32_t listed;
+    explicit ProjectionOverflow(int32_t n)
+        : std::runtime_error("MeshEnergy: projection store overflow"), listed(n) {}
+};
 
 extern thread_local std::string g_last_error;
 

@@ -31,6 +31,7 @@ enum StatusBits : unsigned {
     kStatusNonFiniteStep = 32u, // non-finite deformation-gradient increment     -> domain_error
     kStatusNonFiniteFp = 64u,   // a per-call plastic part with a non-finite entry -> invalid_argument
     kStatusBadDetFp = 128u,     // a per-call plastic part with det <= 0 or non-finite -> domain_error
+    kStatusProjOverflow = 256u, // more listed tets than the projection store holds -> grow it and evaluate again
 };
 
 // Device layout (no per-tet geometry is stored; Dm^-1 and w are recomputed from the rest positions):
@@ -135,7 +136,8 @@ struct EvalArgs {
     unsigned* counter = nullptr; // indefinite count (device)
     int32_t* proj_slot = nullptr; // T
     int32_t* proj_list = nullptr; // T
-    double* proj_H = nullptr;    // T x kProjPerTet (list-indexed, first *counter used)
+    double* proj_H = nullptr;    // proj_capacity x kProjPerTet (list-indexed, first *counter used)
+    int32_t proj_capacity = 0;   // list entries proj_H holds; a longer list sets kStatusProjOverflow
 };
 
 // Projection of a listed (not certified positive definite) tet as a low-rank term: proj(H_e) = (replace ? 0 : H_e)

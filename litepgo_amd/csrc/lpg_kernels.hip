@@ -308,6 +308,7 @@ struct AssembleParams {
     const int32_t* proj_slot;
     const double* proj_H;
     const double* fp = nullptr; // plastic parts (10 per tet) or null
+    int proj_capacity = 0;      // list entries proj_H holds (an overflowing evaluation is repeated by the host)
 };
 
 // Rest and current positions of a vertex.
@@ -567,7 +568,7 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
             // Projected: a listed tet's low-rank term is added by the second pass below; its exact blocks are added
             // here unless the term replaces them (slot <= -2).
             const bool add = KIND == 0 || (KIND == 1 && slot >= -1);
-            if (KIND == 1 && slot != -1) {
+            if (KIND == 1 && slot != -1 && (slot >= 0 ? slot : -2 - slot) < p.proj_capacity) {
                 const int li = slot >= 0 ? slot : -2 - slot;
                 if (pend_s0 < 0) {
                     pend_s0 = li;
@@ -659,6 +660,8 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
                         r = inc[k];
                         const int slot = p.proj_slot[r.tet];
                         li = slot >= 0 ? slot : (slot == -1 ? -1 : -2 - slot);
+                        if (li >= p.proj_capacity)
+                            li = -1;
                     }
                 }
                 double pv[6 + 27];
@@ -917,6 +920,7 @@ struct ProjectParams {
     int32_t* proj_list;
     double* proj_H;
     const double* fp = nullptr; // plastic parts (10 per tet) or null
+    int proj_capacity = 0;      // list entries proj_H holds
 };
 
 __device__ __forceinline__ int sym_index(int r, int c) {
@@ -1030,6 +1034,12 @@ __global__ __launch_bounds__(64, 2) void k_project_jacobi(ProjectParams p) {
         const unsigned idx = blockIdx.x * 64 + threadIdx.x;
         if (idx >= count)
             return;
+        if (idx >= static_cast<unsigned>(p.proj_capacity)) {
+            // The store is too small for this evaluation's list: the host grows it and evaluates again.
+            if (idx == static_cast<unsigned>(p.proj_capacity))
+                atomicOr(p.status, kStatusProjOverflow);
+            return;
+        }
         const int e = p.proj_list[idx];
         double M[45];
         reduced_hessian(p, e, M);
@@ -1940,6 +1950,7 @@ void launch_assemble(const DeviceMesh& m, const EvalArgs& a, const LaunchConfig&
     p.status = a.status;
     p.proj_slot = a.proj_slot;
     p.proj_H = a.proj_H;
+    p.proj_capacity = a.proj_capacity;
     p.fp = m.fp;
     const size_t bytes = assemble_lds_bytes(lc, caps, global_acc, a.kind);
     if (bytes > assemble_lds_limit(lc))
@@ -1982,6 +1993,7 @@ void launch_project_prepass(const DeviceMesh& m, const EvalArgs& a, hipStream_t 
         return;
     ProjectParams p{m.Xr, m.conn, m.theta, a.x, a.theta, m.num_tets, a.status, a.counter, a.proj_slot, a.proj_list, a.proj_H};
     p.fp = m.fp;
+    p.proj_capacity = a.proj_capacity;
     hipLaunchKernelGGL(k_project_prepass, dim3((m.num_tets + kPrepassThreads - 1) / kPrepassThreads),
                        dim3(kPrepassThreads), 0, stream, p);
     check_launch("k_project_prepass");
@@ -1992,6 +2004,7 @@ void launch_project_jacobi(const DeviceMesh& m, const EvalArgs& a, hipStream_t s
         return;
     ProjectParams p{m.Xr, m.conn, m.theta, a.x, a.theta, m.num_tets, a.status, a.counter, a.proj_slot, a.proj_list, a.proj_H};
     p.fp = m.fp;
+    p.proj_capacity = a.proj_capacity;
     // One lane per element that could be listed: the list length is only known on the device, so the workgroups
     // past it exit at once (the listed ones come first).
     const int blocks = (m.num_tets + 63) / 64;

@@ -372,3 +372,25 @@ def test_plain_large_config_full_size(oracle_mod):
         t[axis::3] = 1.0
         assert np.abs(Hm @ t).max() <= 1e-12 * np.abs(H).max() * 30
     assert abs(g[0::3].sum()) <= 1e-10 * np.abs(g).sum()
+
+
+def test_projection_store_grows_when_more_tets_are_listed(oracle_mod):
+    """The low-rank projection store starts at max(4096, T / 16) entries.  A compressed 10^3 grid lists all 6,000
+    tets: the first evaluation overflows the store, the library grows it and evaluates again, and the result meets
+    the oracle bar; graph mode then captures the grown store, and a milder state reuses it."""
+    rng = np.random.default_rng(23)
+    X, tets = kuhn_grid(10)
+    lam, mu = lame_parameters(1e5, 0.45)
+    theta = np.tile([lam, mu], (len(tets), 1))
+    dev, ref = make(oracle_mod, X, tets, theta)
+    u = (-0.4 * X + 0.002 * rng.uniform(-1, 1, X.shape)).reshape(-1)
+    check_against_oracle(dev, ref, u)
+    assert dev.last_indefinite_count == len(tets)
+    dev.set_graph(True)
+    first = dev.eval_values(u, True, "projected")
+    for _ in range(3):
+        again = dev.eval_values(u, True, "projected")
+        assert again[0] == first[0] and np.array_equal(again[1], first[1]) and np.array_equal(again[2], first[2])
+    dev.set_graph(False)
+    u2 = (0.002 * rng.uniform(-1, 1, X.shape)).reshape(-1)
+    check_against_oracle(dev, ref, u2)
