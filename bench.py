@@ -8,8 +8,8 @@ gradient and the PSD-projected Hessian assembled into the CSR value array, with 
 
 Also reported: the assembly kernel against the HBM roofline (HIP events around the kernel on its stream) and its
 FP64 rate against the FP64 vector peak, the per-phase times, the same step with the Exact Hessian, the 256-mesh
-batched evaluation (one autotuned persistent launch sequence replayed as a hipGraph) against a per-mesh launch
-loop, the 32^3 differentiation workload's backward time against its forward time, the equilibrium-sensitivity
+batched evaluation (autotuned launch shape, replayed as a hipGraph) with its own assembly roofline, against a
+per-mesh loop of synchronous calls and one of enqueued calls with a single wait, the 32^3 differentiation workload's backward time against its forward time, the equilibrium-sensitivity
 problem (device Newton + Jacobi-PCG + adjoint on a 48^3 cantilever; 8 uneven beams with --equilibrium-batch), and
 the oracle (C++ CPU restatement of the reference algorithm) timed on this host's cores.
 
@@ -132,8 +132,9 @@ def time_steps(dev, step, kind, steps, warmup):
 
 
 def bench_batched(steps, warmup):
-    """256 uneven meshes: one batched evaluation (autotuned launch, captured hipGraph) against a loop that
-    evaluates each mesh with its own handle, E+g+H(Projected)."""
+    """256 uneven meshes (4..24 cells per axis), E+g+H(Projected): one batched evaluation (autotuned launch shape,
+    captured hipGraph) against loops that evaluate each mesh with its own handle, synchronously and enqueued on
+    one stream with the waits after the last launch."""
     import numpy as np
     import torch
 
@@ -156,6 +157,16 @@ def bench_batched(steps, warmup):
         batch.eval_device(x, True, "projected", out_E=E, out_g=g, out_H=H)
     torch.cuda.synchronize()
     batch_ms = (time.perf_counter() - t0) / steps * 1e3
+    # Per-phase kernel times of the same evaluation (eager launches with HIP events on the handle's stream).
+    batch.set_graph(False)
+    batch.set_timing(True)
+    for _ in range(steps):
+        batch.eval_device(x, True, "projected", out_E=E, out_g=g, out_H=H)
+    torch.cuda.synchronize()
+    bms, bevals = batch.kernel_times()
+    batch.set_timing(False)
+    bphase = [m / bevals for m in bms]
+    B = algorithmic_bytes(T, batch.num_vertices, batch.nnz)  # the sum of the meshes' own algorithmic bytes
 
     plains = [VolumetricFemEnergy(w.num_vertices, w.tets, w.rest, w.theta) for w in meshes]
     xs = [torch.from_numpy(w.u).cuda() for w in meshes]
@@ -166,20 +177,43 @@ def bench_batched(steps, warmup):
         for p, xm, gm, Hm in zip(plains, xs, gs, Hs):
             p.eval_device(xm, True, "projected", out_g=gm, out_H=Hm)
 
-    for _ in range(warmup):
-        loop()
-    torch.cuda.synchronize()
+    def async_loop():
+        # Every mesh's evaluation enqueued on the current stream, then one wait per handle (the first wait
+        # synchronises the stream; the others read back what is already there).
+        for p, xm, gm, Hm in zip(plains, xs, gs, Hs):
+            p.eval_enqueue(xm, True, "projected", out_g=gm, out_H=Hm)
+        for p in plains:
+            p.eval_wait()
+
+    def timed(fn, n):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n * 1e3
+
     loop_steps = max(2, steps // 4)
-    t0 = time.perf_counter()
-    for _ in range(loop_steps):
-        loop()
-    torch.cuda.synchronize()
-    loop_ms = (time.perf_counter() - t0) / loop_steps * 1e3
-    return {"meshes": len(meshes), "tets": T, "ms_per_step": batch_ms, "tets_per_s": T / (batch_ms * 1e-3),
+    loop_ms = timed(loop, loop_steps)
+    async_ms = timed(async_loop, loop_steps)
+    return {"meshes": len(meshes), "tets": T, "vertices": batch.num_vertices, "hessian_nnz": batch.nnz,
+            "ms_per_step": batch_ms, "tets_per_s": T / (batch_ms * 1e-3),
+            "phases_ms": {"psd_test_and_eigen_clamp": bphase[0], "assembly": bphase[1],
+                          "energy_reduction": bphase[2], "all_kernels": bphase[3]},
+            "roofline": {"bound": "hbm", "kernel": ("k_assemble_persistent" if shape[4] else "k_assemble") +
+                                               " (Projected, gradient), all meshes",
+                         "achieved": B / (bphase[1] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": B / (bphase[1] * 1e-3) / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes": B,
+                         "kernel_ms": bphase[1]},
             "per_mesh_loop_ms": loop_ms, "per_mesh_loop_tets_per_s": T / (loop_ms * 1e-3),
             "speedup_vs_per_mesh_loop": loop_ms / batch_ms,
+            "per_mesh_async_loop_ms": async_ms, "per_mesh_async_loop_tets_per_s": T / (async_ms * 1e-3),
+            "speedup_vs_per_mesh_async_loop": async_ms / batch_ms,
             "launch": {"lanes_per_vertex": shape[0], "waves": shape[1], "vertices_per_tile": shape[2],
-                       "lds_budget": shape[3], "persistent_grid": shape[4], "graph": True}}
+                       "lds_budget": shape[3], "persistent_grid": shape[4], "graph": True,
+                       "autotune_ms": tuned_ms}}
 
 
 def bench_differentiation(steps, warmup):

@@ -33,6 +33,20 @@ lpg_status lpg_tet_batch_eval_device(lpg_tet_energy* energy, const double* x_dev
                                      int32_t want_gradient, int32_t hessian_kind, double* energies_dev,
                                      double* gradient_dev, double* hessian_values_dev, void* stream);
 
+/* The evaluation of lpg_tet_batch_eval_device / lpg_tet_energy_eval_device split in two, so that a caller can
+ * enqueue the evaluations of many handles (one per mesh, say) on one stream and synchronise once.  No counterpart
+ * in the reference, whose MeshEnergy::evaluate is synchronous (src/core/energy/scalar/mesh_energy.cpp:30-42).
+ * _enqueue checks the arguments, enqueues the kernels and the read-back of the status and returns without
+ * synchronising; energies_dev as lpg_tet_batch_eval_device (null: the value is reported by _wait only).  The
+ * handle stays busy (other calls fail with LPG_BUSY) until lpg_tet_energy_eval_wait, which synchronises the stream,
+ * maps the status to an error as the synchronous call does, and, when energies_dev was null, stores the energy
+ * (for a batch handle, the sum over its meshes) in *value (may be null; not written when energies_dev was given).
+ * The buffers must stay valid until then.  Graph mode does not apply. */
+lpg_status lpg_tet_energy_eval_enqueue(lpg_tet_energy* energy, const double* x_dev, const double* theta_dev,
+                                       int32_t want_gradient, int32_t hessian_kind, double* energies_dev,
+                                       double* gradient_dev, double* hessian_values_dev, void* stream);
+lpg_status lpg_tet_energy_eval_wait(lpg_tet_energy* energy, double* value);
+
 /* Per-mesh feasible steps: alphas_dev[num_meshes] (device) receives each mesh's max_feasible_step of x along dx
  * (lpg_tet_energy_max_feasible_step restricted to the mesh's tets).  Synchronises on `stream` to read the status. */
 lpg_status lpg_tet_batch_max_feasible_step_device(lpg_tet_energy* energy, const double* x_dev, const double* dx_dev,

@@ -334,9 +334,17 @@ void collect_timing(lpg_tet_energy* h) {
     ++h->timed_evals;
 }
 
+double read_back_status(lpg_tet_energy* h, bool want_counter);
+
 namespace {
 
 double read_back(lpg_tet_energy* h, bool want_counter) {
+    return read_back_status(h, want_counter);
+}
+
+} // namespace
+
+double read_back_status(lpg_tet_energy* h, bool want_counter) {
     double value;
     unsigned status, counter;
     std::memcpy(&value, h->pinned, 8);
@@ -350,9 +358,9 @@ double read_back(lpg_tet_energy* h, bool want_counter) {
     return value;
 }
 
+namespace {
 double run_eval_once(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g,
                      double* H, double* energies, hipStream_t stream);
-
 } // namespace
 
 double run_eval(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g, double* H,
@@ -780,6 +788,56 @@ lpg_status lpg_tet_energy_kernel_times(const lpg_tet_energy* h, double* ms, int3
 
 int32_t lpg_tet_energy_max_degree(const lpg_tet_energy* h) {
     return h ? h->setup.max_degree : 0;
+}
+
+lpg_status lpg_tet_energy_eval_enqueue(lpg_tet_energy* h, const double* x_dev, const double* theta_dev,
+                                       int32_t want_gradient, int32_t hessian_kind, double* energies_dev,
+                                       double* gradient_dev, double* hessian_values_dev, void* stream) {
+    return guarded([&] {
+        if (!h)
+            fail(lpg::kInvalid, "lpg_tet_energy_eval_enqueue: null handle");
+        // The handle stays busy until lpg_tet_energy_eval_wait.
+        bool expected = false;
+        if (!h->busy.compare_exchange_strong(expected, true))
+            fail(lpg::kBusy, "MeshEnergy is already in use by another call");
+        try {
+            const hipStream_t s = static_cast<hipStream_t>(stream);
+            lpg::enqueue_eval(h, x_dev, theta_dev, want_gradient, hessian_kind, gradient_dev, hessian_values_dev,
+                              energies_dev, s);
+            lpg::hip_check(hipMemcpyAsync(h->pinned, h->d_scalars.ptr, 16, hipMemcpyDeviceToHost, s), "read-back");
+            h->pending_eval = {true, x_dev, theta_dev, want_gradient, hessian_kind, gradient_dev, hessian_values_dev,
+                               energies_dev, s};
+        } catch (...) {
+            h->busy.store(false);
+            throw;
+        }
+    });
+}
+
+lpg_status lpg_tet_energy_eval_wait(lpg_tet_energy* h, double* value) {
+    return guarded([&] {
+        if (!h)
+            fail(lpg::kInvalid, "lpg_tet_energy_eval_wait: null handle");
+        if (!h->pending_eval.active)
+            fail(lpg::kLogic, "lpg_tet_energy_eval_wait: no evaluation was enqueued on this handle");
+        const auto e = h->pending_eval;
+        h->pending_eval.active = false;
+        // Release the handle however the wait ends.
+        struct Release {
+            std::atomic<bool>& f;
+            ~Release() { f.store(false); }
+        } release{h->busy};
+        double v;
+        try {
+            lpg::hip_check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
+            v = lpg::read_back_status(h, e.kind == 1);
+        } catch (const lpg::ProjectionOverflow&) {
+            // The projection store was too small for this state: evaluate again with a grown store (synchronous).
+            v = lpg::run_eval(h, e.x, e.theta, e.want_g, e.kind, e.g, e.H, e.energies, e.stream);
+        }
+        if (value && !e.energies)
+            *value = v; // with an energies buffer the energies are there only
+    });
 }
 
 } // extern "C"

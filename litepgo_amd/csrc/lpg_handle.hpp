@@ -105,6 +105,15 @@ struct lpg_tet_energy {
     std::atomic<bool> busy{false};
     int32_t last_indefinite = 0;
     int32_t proj_capacity = 0; // list entries d_proj_H holds (grown when an evaluation lists more tets)
+    // An evaluation enqueued by lpg_tet_energy_eval_enqueue and not yet waited for (the handle stays busy).
+    struct PendingEval {
+        bool active = false;
+        const double* x = nullptr;
+        const double* theta = nullptr;
+        int want_g = 0, kind = -1;
+        double *g = nullptr, *H = nullptr, *energies = nullptr;
+        hipStream_t stream = nullptr;
+    } pending_eval;
     void* pinned = nullptr; // energy + status + counter read-back
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -154,6 +163,8 @@ void collect_timing(lpg_tet_energy* h);
 double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter);
 // The status mapping alone (status word as read back).
 void raise_status(unsigned status);
+// The value, counter and status already read back into the handle's pinned slot (after a synchronisation).
+double read_back_status(lpg_tet_energy* h, bool want_counter);
 // Raised by finish()/read-back when the projection store was too small; run_eval grows it and evaluates again.
 struct ProjectionOverflow : std::runtime_error {
     int32_t listed;

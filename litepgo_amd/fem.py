@@ -307,6 +307,41 @@ class VolumetricFemEnergy:
                       "eval_device")
         return E, g, H
 
+    def eval_enqueue(self, x, gradient=False, hessian=None, theta=None, out_E=None, out_g=None, out_H=None):
+        """Enqueues an evaluation on the current stream without synchronising and returns (E, g, H) tensors that
+        are valid after eval_wait(); E (one entry per mesh of a batch, one for a plain energy) may be None for a
+        plain energy, whose value eval_wait returns.  The handle rejects other calls until eval_wait."""
+        import torch
+
+        self._check_tensor(x, self.num_dofs, "x")
+        if theta is not None:
+            self._check_tensor(theta, 2 * self.num_tets, "theta")
+        kind = kind_code(hessian)
+        if out_E is not None:
+            self._check_tensor(out_E, getattr(self, "num_meshes", 1), "out_E")
+        g = None
+        if gradient:
+            g = out_g if out_g is not None else torch.empty(self.num_dofs, dtype=torch.float64, device=x.device)
+        H = None
+        if kind >= 0:
+            H = out_H if out_H is not None else torch.empty(self.nnz, dtype=torch.float64, device=x.device)
+        _native.check(_native.lib().lpg_tet_energy_eval_enqueue(self._h, _tptr(x), _tptr(theta),
+                                                                int(bool(gradient)), kind, _tptr(out_E), _tptr(g),
+                                                                _tptr(H), _stream()), "eval_enqueue")
+        self._pending = (x, theta, out_E, g, H)  # the buffers stay alive until eval_wait
+        return out_E, g, H
+
+    def eval_wait(self):
+        """Waits for the evaluation enqueued by eval_enqueue and raises its error if any; returns the energy (a
+        batch's total), or None when the energies went to out_E."""
+        value = np.zeros(1)
+        with_E = self._pending is not None and self._pending[2] is not None
+        try:
+            _native.check(_native.lib().lpg_tet_energy_eval_wait(self._h, _ptr(value)), "eval_wait")
+        finally:
+            self._pending = None
+        return None if with_E else float(value[0])
+
     def vjp_device(self, x, gE=None, w=None, theta=None, want_x=True, want_theta=True):
         """(gx, gtheta) = (s f + H w, s dE/dtheta + (df/dtheta)^T w), s the one-entry device tensor gE (0 if
         None), w a num_dofs device tensor (0 if None); either output is None when not wanted."""

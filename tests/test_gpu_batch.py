@@ -123,3 +123,32 @@ def test_error_in_one_mesh_fails_the_batch(torch, meshes):
         batch.eval_device(x, True, "exact")
     # the handle recovers
     batch.eval_device(union_x(torch, meshes), True, "exact")
+
+
+def test_enqueued_per_mesh_evaluations_equal_synchronous_ones(torch, meshes):
+    """Evaluations of many plain handles enqueued on one stream and waited for afterwards give the bits of the
+    synchronous calls; the handle is busy in between; waiting twice fails; a batch handle reports per-mesh
+    energies through the same pair."""
+    from litepgo_amd._native import BusyError
+
+    handles = [VolumetricFemEnergy(w.num_vertices, w.tets, w.rest, w.theta) for w in meshes]
+    xs = [torch.from_numpy(w.u.copy()).cuda() for w in meshes]
+    ref = [h.eval_device(x, True, "projected") for h, x in zip(handles, xs)]
+    outs = [h.eval_enqueue(x, True, "projected") for h, x in zip(handles, xs)]
+    with pytest.raises(BusyError):
+        handles[0].eval_device(xs[0], True, "projected")
+    values = [h.eval_wait() for h in handles]
+    for m, ((v, g, H), (_, g2, H2), v2) in enumerate(zip(ref, outs, values)):
+        assert v2 == v and torch.equal(g2, g) and torch.equal(H2, H), m
+    with pytest.raises(RuntimeError, match="no evaluation was enqueued"):
+        handles[0].eval_wait()
+    batch = BatchedFemEnergy(meshes)
+    x = union_x(torch, meshes)
+    E0, g0, H0 = batch.eval_device(x, True, "exact")
+    E = torch.empty_like(E0)
+    _, g, H = batch.eval_enqueue(x, True, "exact", out_E=E)
+    assert batch.eval_wait() is None
+    assert torch.equal(E, E0) and torch.equal(g, g0) and torch.equal(H, H0)
+    batch.eval_enqueue(x, False, None)
+    total = batch.eval_wait()  # no energies buffer: the sum over the meshes
+    assert abs(total - float(E0.sum())) <= TOL * float(E0.abs().sum())
