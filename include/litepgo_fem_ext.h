@@ -41,6 +41,22 @@ lpg_status lpg_lame_device(const double* E_dev, const double* nu_dev, int64_t n,
 lpg_status lpg_lame_vjp_device(const double* E_dev, const double* nu_dev, const double* gtheta_dev, int64_t n,
                                double* gE_dev, double* gnu_dev, void* stream);
 
+/* lpg_tet_energy_eval_device without any device allocation, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  Arguments,
+ * results (bit for bit) and errors are those of lpg_tet_energy_eval_device; the per-evaluation scratch (status and
+ * energy slot, vertex energies, reduction partials and, for the Projected Hessian, the projection slots, list and
+ * low-rank store) comes from one static 64 MiB device array per GPU, held until the process ends, instead of the
+ * handle's buffers.  Limits: the handle's mesh must already be on the device (one evaluation through an allocating
+ * entry point, or lpg_tet_energy_set_launch, puts it there; LPG_LOGIC_ERROR otherwise); a mesh whose scratch
+ * exceeds 64 MiB (about 8 bytes per vertex, plus 8 per tet and 416 per tet that needs the eigen-analysis with the
+ * Projected Hessian) fails with LPG_INVALID_ARGUMENT, as does an evaluation that lists more tets for the
+ * eigen-analysis than the array holds.  Every call on one GPU uses the same array, whatever the handle: calls that
+ * overlap in time on one GPU (other streams, other threads) race; calls in sequence on one stream are safe.  Graph
+ * mode does not apply (the evaluation is enqueued eagerly and synchronised once to read the status). */
+lpg_status lpg_tet_energy_eval_device_noalloc(lpg_tet_energy* energy, const double* x_dev, const double* theta_dev,
+                                              int32_t want_gradient, int32_t hessian_kind, double* value,
+                                              double* gradient_dev, double* hessian_values_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

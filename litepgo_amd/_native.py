@@ -48,6 +48,7 @@ SIGNATURES = {
     "lpg_tet_batch_max_feasible_step_device": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "lpg_tet_energy_eval_enqueue": (ctypes.c_int, [_P, _P, _P, _I32, _I32, _P, _P, _P, _P]),
     "lpg_tet_energy_eval_wait": (ctypes.c_int, [_P, _P]),
+    "lpg_tet_energy_eval_device_noalloc": (ctypes.c_int, [_P, _P, _P, _I32, _I32, _P, _P, _P, _P]),
     "lpg_tet_energy_set_graph": (ctypes.c_int, [_P, _I32]),
     "lpg_tet_energy_autotune": (ctypes.c_int, [_P, _P, _I32, _I32, _P, _P]),
     "lpg_newton_default_options": (None, [_P]),

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <thread>
 
 namespace lpg {
@@ -233,8 +234,9 @@ void raise_status(unsigned status) {
         fail(kDomain, "VolumetricFemEnergy: non-finite input increment");
 }
 
-double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter) {
-    hip_check(hipMemcpyAsync(h->pinned, h->d_scalars.ptr, 16, hipMemcpyDeviceToHost, stream), "read-back");
+double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter, const void* scalars) {
+    hip_check(hipMemcpyAsync(h->pinned, scalars ? scalars : h->d_scalars.ptr, 16, hipMemcpyDeviceToHost, stream),
+              "read-back");
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     double value;
     unsigned status, counter;
@@ -250,7 +252,7 @@ double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter) {
 }
 
 void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g, double* H,
-                  double* energies, hipStream_t stream) {
+                  double* energies, hipStream_t stream, const EvalScratch* ws) {
     if (kind < -1 || kind > 1)
         fail(kInvalid, "Energy: invalid Hessian kind");
     if ((want_g && !g && h->setup.num_dofs > 0) || (kind >= 0 && !H && h->setup.nnz > 0) ||
@@ -258,7 +260,32 @@ void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int w
         fail(kInvalid, "lpg_tet_energy_eval: missing input or output buffer");
     if (!theta && !h->setup.theta_valid && h->setup.num_tets > 0)
         fail(kInvalid, "NeoHookean requires finite lambda >= 0 and mu > 0");
-    ensure_device(h);
+    if (ws && !h->device_ready)
+        fail(kLogic, "the mesh of this handle is not on the device yet: evaluate it once through an allocating "
+                     "entry point first");
+    ensure_device(h); // (only selects the device once the handle is there)
+    EvalScratch own;
+    if (!ws) {
+        own.scalars = h->d_scalars.ptr;
+        own.vertex_E = h->d_vertex_E.as<double>();
+        own.chunk_sum = h->d_chunk_sum.as<double>();
+        own.seg_energy = h->d_seg_energy.as<double>();
+        if (kind == 1) {
+            const size_t T = static_cast<size_t>(h->setup.num_tets);
+            h->d_proj_slot.ensure(std::max<size_t>(1, T) * sizeof(int32_t));
+            h->d_proj_list.ensure(std::max<size_t>(1, T) * sizeof(int32_t));
+            // The low-rank store holds the listed (not certified positive definite) tets only: 1/16 of the tets to
+            // start with, grown to the listed count when an evaluation lists more (run_eval repeats it then).
+            if (h->proj_capacity == 0)
+                h->proj_capacity = static_cast<int32_t>(std::min<size_t>(T, std::max<size_t>(4096, T / 16)));
+            h->d_proj_H.ensure(std::max<size_t>(1, h->proj_capacity) * kProjPerTet * sizeof(double));
+            own.proj_slot = h->d_proj_slot.as<int32_t>();
+            own.proj_list = h->d_proj_list.as<int32_t>();
+            own.proj_H = h->d_proj_H.as<double>();
+            own.proj_capacity = h->proj_capacity;
+        }
+        ws = &own;
+    }
     EvalArgs a;
     a.x = x;
     a.theta = theta;
@@ -266,24 +293,16 @@ void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int w
     a.H = kind >= 0 ? H : nullptr;
     a.want_g = want_g;
     a.kind = kind;
-    a.vertex_E = h->d_vertex_E.as<double>();
-    a.energy = h->d_energy();
-    a.status = h->d_status();
-    a.counter = h->d_counter();
+    a.vertex_E = ws->vertex_E;
+    a.energy = static_cast<double*>(ws->scalars);
+    a.status = reinterpret_cast<unsigned*>(static_cast<char*>(ws->scalars) + 8);
+    a.counter = reinterpret_cast<unsigned*>(static_cast<char*>(ws->scalars) + 12);
     const DeviceMesh m = h->mesh();
     if (kind == 1) {
-        const size_t T = static_cast<size_t>(h->setup.num_tets);
-        h->d_proj_slot.ensure(std::max<size_t>(1, T) * sizeof(int32_t));
-        h->d_proj_list.ensure(std::max<size_t>(1, T) * sizeof(int32_t));
-        // The low-rank store holds the listed (not certified positive definite) tets only: 1/16 of the tets to
-        // start with, grown to the listed count when an evaluation lists more (run_eval repeats it then).
-        if (h->proj_capacity == 0)
-            h->proj_capacity = static_cast<int32_t>(std::min<size_t>(T, std::max<size_t>(4096, T / 16)));
-        h->d_proj_H.ensure(std::max<size_t>(1, h->proj_capacity) * kProjPerTet * sizeof(double));
-        a.proj_slot = h->d_proj_slot.as<int32_t>();
-        a.proj_list = h->d_proj_list.as<int32_t>();
-        a.proj_H = h->d_proj_H.as<double>();
-        a.proj_capacity = h->proj_capacity;
+        a.proj_slot = ws->proj_slot;
+        a.proj_list = ws->proj_list;
+        a.proj_H = ws->proj_H;
+        a.proj_capacity = ws->proj_capacity;
     }
     if (h->timing)
         hip_check(hipEventRecord(h->ev[0], stream), "hipEventRecord");
@@ -309,10 +328,9 @@ void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int w
     const int segments = static_cast<int>(h->seg_chunk.size()) - 1;
     // Per-segment energies go to `energies`; without it, a batch's segment energies go to scratch and their total
     // to the handle's energy slot (the single-energy contract of the union).
-    double* seg_out = energies ? energies : (segments > 1 ? h->d_seg_energy.as<double>() : a.energy);
+    double* seg_out = energies ? energies : (segments > 1 ? ws->seg_energy : a.energy);
     launch_energy_reduce(a.vertex_E, h->d_chunk_begin.as<int32_t>(), static_cast<int>(h->chunk_begin.size()) - 1,
-                         h->d_chunk_sum.as<double>(), h->d_seg_chunk.as<int32_t>(), segments, seg_out, a.status,
-                         stream);
+                         ws->chunk_sum, h->d_seg_chunk.as<int32_t>(), segments, seg_out, a.status, stream);
     if (!energies && segments > 1)
         launch_energy_total(seg_out, segments, a.energy, stream);
     if (h->timing)
@@ -837,6 +855,122 @@ lpg_status lpg_tet_energy_eval_wait(lpg_tet_energy* h, double* value) {
         }
         if (value && !e.energies)
             *value = v; // with an energies buffer the energies are there only
+    });
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------------------------------------------
+// Allocation-free evaluation: the per-evaluation scratch (status and energy slot, vertex energies, reduction
+// partials, projection slots, list and low-rank store) lives in one static device array per GPU instead of the
+// handle's allocated buffers.
+// ---------------------------------------------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = size_t{64} << 20; // 64 MiB per device, held until the process ends
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {}; // device address of `scratch` on each device (null until looked up)
+
+// The device address of `scratch` on `device`, looked up once per device with that device current.  The host-side
+// name `scratch` is not a device address; only the pointer looked up here is passed to kernels and copies.
+char* scratch_on(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (device < 0 || device >= kMaxDevices)
+        lpg::fail(lpg::kDevice, "device index out of range for the allocation-free scratch");
+    if (!g_scratch_ptr[device]) {
+        int current = 0;
+        lpg::hip_check(hipGetDevice(&current), "hipGetDevice");
+        if (current != device)
+            lpg::hip_check(hipSetDevice(device), "hipSetDevice");
+        void* ptr = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (current != device)
+            (void)hipSetDevice(current);
+        lpg::hip_check(err, "hipGetSymbolAddress(scratch)");
+        if (!ptr)
+            lpg::fail(lpg::kDevice, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_ptr[device] = ptr;
+    }
+    return static_cast<char*>(g_scratch_ptr[device]);
+}
+
+size_t round256(size_t n) {
+    return (n + 255) & ~size_t{255};
+}
+
+} // namespace
+
+extern "C" {
+
+lpg_status lpg_tet_energy_eval_device_noalloc(lpg_tet_energy* h, const double* x_dev, const double* theta_dev,
+                                              int32_t want_gradient, int32_t hessian_kind, double* value,
+                                              double* gradient_dev, double* hessian_values_dev, void* stream) {
+    return guarded([&] {
+        if (!h)
+            fail(lpg::kInvalid, "lpg_tet_energy_eval_device_noalloc: null handle");
+        if (!value)
+            fail(lpg::kInvalid, "lpg_tet_energy_eval: value pointer is null");
+        lpg::BusyScope scope(h->busy);
+        if (!h->device_ready)
+            fail(lpg::kLogic, "lpg_tet_energy_eval_device_noalloc: the mesh of this handle is not on the device "
+                              "yet: evaluate it once through an allocating entry point first");
+        const size_t V = static_cast<size_t>(std::max(1, h->setup.num_vertices));
+        const size_t T = static_cast<size_t>(std::max(1, h->setup.num_tets));
+        const size_t chunks = std::max<size_t>(1, h->chunk_begin.size());
+        const size_t segs = std::max<size_t>(1, h->seg_chunk.size());
+        // Layout: scalars | vertex energies | chunk sums | segment energies | [slots | list | low-rank store].
+        size_t off = 256;
+        const size_t o_vE = off;
+        off += round256(V * sizeof(double));
+        const size_t o_chunk = off;
+        off += round256(chunks * sizeof(double));
+        const size_t o_seg = off;
+        off += round256(segs * sizeof(double));
+        size_t o_slot = 0, o_list = 0, o_H = 0;
+        int32_t capacity = 0;
+        if (hessian_kind == 1) {
+            o_slot = off;
+            off += round256(T * sizeof(int32_t));
+            o_list = off;
+            off += round256(T * sizeof(int32_t));
+            o_H = off;
+            const size_t per = lpg::kProjPerTet * sizeof(double);
+            const size_t room = off < kNoAllocScratchBytes ? (kNoAllocScratchBytes - off) / per : 0;
+            capacity = static_cast<int32_t>(std::min(T, room));
+            if (capacity > 0)
+                off += capacity * per;
+            else
+                off += per; // reported below as too large
+        }
+        if (off > kNoAllocScratchBytes)
+            fail(lpg::kInvalid, "lpg_tet_energy_eval_device_noalloc: this mesh needs " + std::to_string(off) +
+                                    " bytes of scratch; the allocation-free entry point has " +
+                                    std::to_string(kNoAllocScratchBytes) + " per device");
+        char* base = scratch_on(h->device);
+        lpg::EvalScratch ws;
+        ws.scalars = base;
+        ws.vertex_E = reinterpret_cast<double*>(base + o_vE);
+        ws.chunk_sum = reinterpret_cast<double*>(base + o_chunk);
+        ws.seg_energy = reinterpret_cast<double*>(base + o_seg);
+        if (hessian_kind == 1) {
+            ws.proj_slot = reinterpret_cast<int32_t*>(base + o_slot);
+            ws.proj_list = reinterpret_cast<int32_t*>(base + o_list);
+            ws.proj_H = reinterpret_cast<double*>(base + o_H);
+            ws.proj_capacity = capacity;
+        }
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        lpg::enqueue_eval(h, x_dev, theta_dev, want_gradient, hessian_kind, gradient_dev, hessian_values_dev, nullptr,
+                          s, &ws);
+        try {
+            *value = lpg::finish(h, s, hessian_kind == 1, base);
+        } catch (const lpg::ProjectionOverflow& o) {
+            fail(lpg::kInvalid, "lpg_tet_energy_eval_device_noalloc: " + std::to_string(o.listed) +
+                                    " tets need the eigen-analysis, the scratch holds " + std::to_string(capacity));
+        }
     });
 }
 

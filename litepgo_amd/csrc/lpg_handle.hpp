@@ -143,12 +143,26 @@ bool launch_config_ok(const lpg_tet_energy* h, const LaunchConfig& lc, int verti
 // (Re)builds the tiles for a launch configuration.
 void set_plan(lpg_tet_energy* h, const LaunchConfig& lc, int vertices_per_tile, int lds_budget,
               int persistent_grid);
+// Device memory an evaluation uses besides its inputs and outputs: by default the handle's own buffers (allocated
+// or grown on demand), or memory the caller lays out (the allocation-free entry point).
+struct EvalScratch {
+    void* scalars = nullptr;      // energy (double) + status + counter: 16 bytes
+    double* vertex_E = nullptr;   // num_vertices
+    double* chunk_sum = nullptr;  // energy reduction chunks
+    double* seg_energy = nullptr; // per-segment energies of a batch evaluated as one energy
+    int32_t* proj_slot = nullptr; // num_tets (Projected only)
+    int32_t* proj_list = nullptr; // num_tets (Projected only)
+    double* proj_H = nullptr;     // proj_capacity * kProjPerTet (Projected only)
+    int32_t proj_capacity = 0;
+};
 // Checks arguments and enqueues one evaluation (no synchronisation).  energies (device) receives one energy per
-// vertex segment (seg_vbeg); null means the handle's own scalar slot (plain energies).
+// vertex segment (seg_vbeg); null means the scalar slot (plain energies).  ws: the scratch to use instead of the
+// handle's buffers (then nothing is allocated, and the handle must already be on the device).
 void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g, double* H,
-                  double* energies, hipStream_t stream);
-// Enqueues the read-back of status, counter and the first energy, synchronises, maps the status to an error.
-double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter);
+                  double* energies, hipStream_t stream, const EvalScratch* ws = nullptr);
+// Enqueues the read-back of status, counter and the first energy from `scalars` (null: the handle's own slot),
+// synchronises, maps the status to an error.
+double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter, const void* scalars = nullptr);
 // One evaluation (eager, or replayed from the handle's captured graph when graph mode is on and the call repeats);
 // returns the first energy (the plain energy; per-mesh energies go to `energies` when given).
 double run_eval(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g, double* H,
@@ -159,8 +173,6 @@ double feasible_step_call(lpg_tet_energy* h, const double* x, const double* dx, 
                           hipStream_t s);
 // Accumulates the per-phase event times of the last eager evaluation (timing mode).
 void collect_timing(lpg_tet_energy* h);
-// Reads back the scalars (value, status) enqueued so far on `stream`, synchronises and raises on a bad status.
-double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter);
 // The status mapping alone (status word as read back).
 void raise_status(unsigned status);
 // The value, counter and status already read back into the handle's pinned slot (after a synchronisation).
