@@ -535,6 +535,9 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
         Inc pend_r0{}, pend_r1{};
         int pend_s0 = -1, pend_s1 = -1;
         bool pend_all = false;
+        // Rounds (m = (k - kb - sub) / LPV) of listed incidences after the first two, one bit each; a listed round
+        // at m >= 64 (a vertex with more than 256 incidences) sets `pend_all` instead.
+        unsigned long long pend_more = 0ull;
         const auto clampk = [&](int k) { return k < ke ? k : ke - 1; };
         const auto load_ids = [&](const Inc& r, int (&id)[3]) {
 #pragma unroll
@@ -547,10 +550,10 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
                 slot = p.proj_slot[r.tet];
         };
         // One round: the incidence's values, the register sums, and the off-diagonal blocks into the segment.
-        // Indefinite incidences (Projected) keep the records and projection slots of up to two per lane for the
-        // second pass; more set `pend_all`, which rescans the lane's incidences.
+        // Listed incidences (Projected) keep the records and projection slots of up to two per lane for the
+        // second pass; later ones are remembered by their round index m (bit m of `pend_more`).
         const VertexData me{{Xv.x, Xv.y, Xv.z}, {uv0, uv1, uv2}};
-        const auto round = [&](const Inc& r, const NbrData& d, int slot, bool active) {
+        const auto round = [&](const Inc& r, const NbrData& d, int slot, bool active, int m) {
             if (!active)
                 return;
             int cps[3];
@@ -576,6 +579,8 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
                 } else if (pend_s1 < 0) {
                     pend_s1 = li;
                     pend_r1 = r;
+                } else if (m < 64) {
+                    pend_more |= 1ull << m;
                 } else {
                     pend_all = true;
                 }
@@ -624,14 +629,14 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
                     load_data(r1, id1, dB, sB);
                     load_ids(r2, id2);
                     const Inc r3 = inc[clampk(kc + 3 * LPV)];
-                    round(r0, dA, sA, kc < ke);
+                    round(r0, dA, sA, kc < ke, (k0 - kb) / LPV);
                     if (k0 + LPV >= ke)
                         break;
                     // Round m+1 (buffer B); stages for rounds m+2, m+3, m+4.
                     load_data(r2, id2, dA, sA);
                     load_ids(r3, id3);
                     const Inc r4 = inc[clampk(kc + 4 * LPV)];
-                    round(r1, dB, sB, kc + LPV < ke);
+                    round(r1, dB, sB, kc + LPV < ke, (k0 - kb) / LPV + 1);
                     if (k0 + 2 * LPV >= ke)
                         break;
                     r0 = r2;
@@ -646,85 +651,128 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
         if (KIND == 1) {
             // Second pass: listed tets add their low-rank term sum_v u_v u_v^T (stored by the eigen-clamp kernel):
             // for slot a, block (a, a) (6 distinct entries) and blocks (a, a^j), j = 1..3.  Keeping these loads out
-            // of the first pass keeps them from sharing registers with the exact values there.
-            const int steps = pend_all ? (ke - kb + LPV - 1) / LPV : 2;
-            for (int q = 0; q < steps; ++q) {
-                bool add = false;
-                int cps[3] = {0, 0, 0};
-                Inc r = q == 0 ? pend_r0 : pend_r1;
-                int li = q == 0 ? pend_s0 : pend_s1;
-                if (pend_all) {
-                    const int k = kb + sub + q * LPV;
-                    li = -1;
-                    if (k < ke) {
-                        r = inc[k];
-                        const int slot = p.proj_slot[r.tet];
-                        li = slot >= 0 ? slot : (slot == -1 ? -1 : -2 - slot);
-                        if (li >= p.proj_capacity)
-                            li = -1;
+            // of the first pass keeps them from sharing registers with the exact values there.  Entries go two at a
+            // time: both entries' counts and first two vectors are loaded in one round (one memory latency for the
+            // pair; a count above two loads the remaining vectors afterwards), then entry 0's sums and blocks are
+            // added before entry 1's, so the order of additions is the entries' order.
+            // The first two listed entries are in registers; later ones are taken from `pend_more` in round order
+            // (their records and slots loaded again); `pend_all` rescans every round instead.
+            const int rescan_steps = (ke - kb + 2 * LPV - 1) / (2 * LPV);
+            for (int q = 0;; ++q) {
+                const bool more = pend_all ? q < rescan_steps : (q == 0 || pend_more != 0ull);
+                if (!eig9::any_lane(more))
+                    break;
+                Inc rr[2] = {pend_r0, pend_r1};
+                int li[2] = {q == 0 ? pend_s0 : -1, q == 0 ? pend_s1 : -1};
+                if (pend_all || q > 0) {
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        int k = ke;
+                        if (pend_all) {
+                            k = kb + sub + (2 * q + e) * LPV;
+                        } else if (pend_more != 0ull) {
+                            const int m = __builtin_ctzll(pend_more);
+                            pend_more &= pend_more - 1ull;
+                            k = kb + sub + m * LPV;
+                        }
+                        li[e] = -1;
+                        if (k < ke) {
+                            rr[e] = inc[k];
+                            const int slot = p.proj_slot[rr[e].tet];
+                            li[e] = slot >= 0 ? slot : (slot == -1 ? -1 : -2 - slot);
+                            if (li[e] >= p.proj_capacity)
+                                li[e] = -1;
+                        }
                     }
                 }
-                double pv[6 + 27];
+                // Element DOF positions of the entry's rows: slot a first, then a^1, a^2, a^3.
+                double cnt[2] = {0.0, 0.0};
+                double uv[2][2][12];
 #pragma unroll
-                for (int c = 0; c < 6 + 27; ++c)
-                    pv[c] = 0.0;
-                if (li >= 0) {
-                    add = true;
+                for (int e = 0; e < 2; ++e) {
+                    if (li[e] >= 0) {
+                        const int a = IncT<WIDE>::slot(rr[e]);
+                        const double* src = p.proj_H + static_cast<size_t>(li[e]) * kProjPerTet;
+                        cnt[e] = src[0];
 #pragma unroll
-                    for (int j = 1; j < 4; ++j)
-                        cps[j - 1] = IncT<WIDE>::cp(r, j);
-                    const int a = IncT<WIDE>::slot(r);
-                    const double* src = p.proj_H + static_cast<size_t>(li) * kProjPerTet;
-                    // u_0 is always present (zero when the count is 0), so its loads go out together with the count's;
-                    // further vectors (count > 1) are rare and loaded after it.
-                    double cnt = 0.0;
-                    for (int vv = 0; vv == 0 || vv < static_cast<int>(cnt); ++vv) {
-                        const double* u = src + 4 + 12 * vv;
-                        double ua[3], ub[3][3];
-                        // All loads of a vector (and, for u_0, the count) are issued before any product (the empty
-                        // asm orders them).
-                        if (vv == 0)
-                            cnt = src[0];
+                        for (int vv = 0; vv < 2; ++vv)
 #pragma unroll
-                        for (int i = 0; i < 3; ++i)
-                            ua[i] = u[3 * a + i];
+                            for (int j = 0; j < 4; ++j)
 #pragma unroll
-                        for (int j = 1; j < 4; ++j)
+                                for (int k = 0; k < 3; ++k)
+                                    uv[e][vv][3 * j + k] = src[4 + 12 * vv + 3 * (a ^ j) + k];
+                    } else {
 #pragma unroll
-                            for (int k = 0; k < 3; ++k)
-                                ub[j - 1][k] = u[3 * (a ^ j) + k];
-                        asm volatile("" ::: "memory");
-                        pv[0] = fma(ua[0], ua[0], pv[0]);
-                        pv[1] = fma(ua[0], ua[1], pv[1]);
-                        pv[2] = fma(ua[0], ua[2], pv[2]);
-                        pv[3] = fma(ua[1], ua[1], pv[3]);
-                        pv[4] = fma(ua[1], ua[2], pv[4]);
-                        pv[5] = fma(ua[2], ua[2], pv[5]);
+                        for (int vv = 0; vv < 2; ++vv)
+#pragma unroll
+                            for (int i = 0; i < 12; ++i)
+                                uv[e][vv][i] = 0.0;
+                    }
+                }
+                asm volatile("" ::: "memory");
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const bool add = li[e] >= 0;
+                    double pv[6 + 27];
+#pragma unroll
+                    for (int c = 0; c < 6 + 27; ++c)
+                        pv[c] = 0.0;
+                    const auto outer = [&](const double (&u)[12]) {
+                        pv[0] = fma(u[0], u[0], pv[0]);
+                        pv[1] = fma(u[0], u[1], pv[1]);
+                        pv[2] = fma(u[0], u[2], pv[2]);
+                        pv[3] = fma(u[1], u[1], pv[3]);
+                        pv[4] = fma(u[1], u[2], pv[4]);
+                        pv[5] = fma(u[2], u[2], pv[5]);
 #pragma unroll
                         for (int j = 0; j < 3; ++j)
 #pragma unroll
                             for (int i = 0; i < 3; ++i)
 #pragma unroll
                                 for (int k = 0; k < 3; ++k)
-                                    pv[6 + 9 * j + 3 * i + k] = fma(ua[i], ub[j][k], pv[6 + 9 * j + 3 * i + k]);
+                                    pv[6 + 9 * j + 3 * i + k] = fma(u[i], u[3 + 3 * j + k], pv[6 + 9 * j + 3 * i + k]);
+                    };
+                    const int n = add ? static_cast<int>(cnt[e]) : 0;
+                    // u_0 is always stored (zero when the count is 0); u_1 only when the count is above one.
+                    outer(uv[e][0]);
+                    if (n > 1)
+                        outer(uv[e][1]);
+                    if (n > 2) {
+                        const int a = IncT<WIDE>::slot(rr[e]);
+                        const double* src = p.proj_H + static_cast<size_t>(li[e]) * kProjPerTet;
+                        for (int vv = 2; vv < n; ++vv) {
+                            double u[12];
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                                for (int k = 0; k < 3; ++k)
+                                    u[3 * j + k] = src[4 + 12 * vv + 3 * (a ^ j) + k];
+                            outer(u);
+                        }
                     }
+                    int cps[3] = {0, 0, 0};
+                    if (add) {
 #pragma unroll
-                    for (int c = 0; c < 6; ++c)
-                        ds[c] += pv[c];
-                }
-                // Block by block, the lanes of a vertex taking turns within a block in HBM accumulation: the
-                // (block, lane) order of the LDS adds.
+                        for (int j = 1; j < 4; ++j)
+                            cps[j - 1] = IncT<WIDE>::cp(rr[e], j);
 #pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    double* blk = row + 3 * cps[j];
+                        for (int c = 0; c < 6; ++c)
+                            ds[c] += pv[c];
+                    }
+                    // Block by block, the lanes of a vertex taking turns within a block in HBM accumulation: the
+                    // (block, lane) order of the LDS adds.
 #pragma unroll
-                    for (int s = 0; s < (GLOBAL_ACC ? LPV : 1); ++s) {
-                        if (add && (!GLOBAL_ACC || sub == s)) {
+                    for (int j = 0; j < 3; ++j) {
+                        double* blk = row + 3 * cps[j];
 #pragma unroll
-                            for (int i = 0; i < 3; ++i)
+                        for (int s = 0; s < (GLOBAL_ACC ? LPV : 1); ++s) {
+                            if (add && (!GLOBAL_ACC || sub == s)) {
 #pragma unroll
-                                for (int kk = 0; kk < 3; ++kk)
-                                    accum<GLOBAL_ACC>(blk + i * rowlen + kk, pv[6 + 9 * j + 3 * i + kk]);
+                                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                                    for (int kk = 0; kk < 3; ++kk)
+                                        accum<GLOBAL_ACC>(blk + i * rowlen + kk, pv[6 + 9 * j + 3 * i + kk]);
+                            }
                         }
                     }
                 }
