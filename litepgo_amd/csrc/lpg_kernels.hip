@@ -505,8 +505,11 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
         for (int i = threadIdx.x; i < seg1 - seg0; i += kThreads)
             acc[i] = 0.0;
     const int lane = static_cast<int>(threadIdx.x) % kWave;
-    const int sub = lane % LPV;
-    const int lv = (static_cast<int>(threadIdx.x) / kWave) * kPerWave + lane / LPV;
+    // Lanes are numbered sub-major (lane = sub * kPerWave + vertex): the LDS serves a 64-bit access 16 lanes at a
+    // time, and with the LPV lanes of one vertex in different groups of 16 their adds to the same block (tets that
+    // share an edge) no longer collide within a group.  Colliding adds still apply in ascending sub order.
+    const int sub = lane / kPerWave;
+    const int lv = (static_cast<int>(threadIdx.x) / kWave) * kPerWave + lane % kPerWave;
     const int v = vb + lv;
     const bool owner = lane < kPerWave * LPV && v < ve;
     __syncthreads();
