@@ -816,7 +816,10 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
     }
     if (KIND >= 0) {
         if (!GLOBAL_ACC) {
-            // Four LDS reads in flight per lane before their stores: one LDS latency per four values.
+            // Four LDS reads in flight per lane before their stores: one LDS latency per four values.  The values are
+            // stored non-temporally: the Hessian (the bulk of the kernel's traffic) is written once and not read
+            // back, and keeping it out of L2 leaves L2 to the vertex data and (lambda, mu) that several incidences
+            // share.
             const int n = seg1 - seg0;
             int i = threadIdx.x;
             for (; i + 3 * kThreads < n; i += 4 * kThreads) {
@@ -828,14 +831,14 @@ __device__ __forceinline__ void assemble_tile(const AssembleParams& p, int tile,
                 for (int q = 0; q < 4; ++q) {
                     if (!isfinite(val[q]))
                         flags |= kStatusNonFiniteOut;
-                    p.H[seg0 + i + q * kThreads] = val[q];
+                    __builtin_nontemporal_store(val[q], p.H + seg0 + i + q * kThreads);
                 }
             }
             for (; i < n; i += kThreads) {
                 const double val = acc[i];
                 if (!isfinite(val))
                     flags |= kStatusNonFiniteOut;
-                p.H[seg0 + i] = val;
+                __builtin_nontemporal_store(val, p.H + seg0 + i);
             }
         } else {
             for (int i = threadIdx.x; i < seg1 - seg0; i += kThreads)
