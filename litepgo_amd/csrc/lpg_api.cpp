@@ -153,8 +153,11 @@ void ensure_device(lpg_tet_energy* h, bool snapshot_fp) {
     hip_check(hipGetDevice(&h->device), "hipGetDevice");
     if (!h->own_stream)
         hip_check(hipStreamCreate(&h->own_stream), "hipStreamCreate");
-    if (!h->pinned)
+    if (!h->pinned) {
         hip_check(hipHostMalloc(&h->pinned, 64, hipHostMallocDefault), "hipHostMalloc");
+        if (hipHostGetDevicePointer(&h->pinned_dev, h->pinned, 0) != hipSuccess)
+            h->pinned_dev = nullptr;
+    }
     const auto& s = h->setup;
     upload(h->d_Xr, s.Xr.data(), s.Xr.size() * sizeof(double));
     upload(h->d_conn, s.conn.data(), s.conn.size() * sizeof(int32_t));
@@ -252,7 +255,7 @@ double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter, const vo
 }
 
 void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g, double* H,
-                  double* energies, hipStream_t stream, const EvalScratch* ws) {
+                  double* energies, hipStream_t stream, const EvalScratch* ws, void* readback) {
     if (kind < -1 || kind > 1)
         fail(kInvalid, "Energy: invalid Hessian kind");
     if ((want_g && !g && h->setup.num_dofs > 0) || (kind >= 0 && !H && h->setup.nnz > 0) ||
@@ -330,7 +333,8 @@ void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int w
     // to the handle's energy slot (the single-energy contract of the union).
     double* seg_out = energies ? energies : (segments > 1 ? ws->seg_energy : a.energy);
     launch_energy_reduce(a.vertex_E, h->d_chunk_begin.as<int32_t>(), static_cast<int>(h->chunk_begin.size()) - 1,
-                         ws->chunk_sum, h->d_seg_chunk.as<int32_t>(), segments, seg_out, a.status, stream);
+                         ws->chunk_sum, h->d_seg_chunk.as<int32_t>(), segments, seg_out, a.status, stream, a.counter,
+                         segments == 1 ? readback : nullptr);
     if (!energies && segments > 1)
         launch_energy_total(seg_out, segments, a.energy, stream);
     if (h->timing)
@@ -400,6 +404,8 @@ namespace {
 double run_eval_once(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g,
                      double* H, double* energies, hipStream_t stream) {
     ensure_device(h);
+    // A single-mesh evaluation hands its scalars to the host from its last kernel (no copy node).
+    void* fused = h->seg_chunk.size() == 2 ? h->pinned_dev : nullptr;
     GraphCache& gc = h->graph;
     if (gc.enabled && !h->timing) {
         GraphKey k;
@@ -417,9 +423,10 @@ double run_eval_once(lpg_tet_energy* h, const double* x, const double* theta, in
             gc.reset();
             hip_check(hipStreamBeginCapture(h->own_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             try {
-                enqueue_eval(h, x, theta, want_g, kind, g, H, energies, h->own_stream);
-                hip_check(hipMemcpyAsync(h->pinned, h->d_scalars.ptr, 16, hipMemcpyDeviceToHost, h->own_stream),
-                          "read-back");
+                enqueue_eval(h, x, theta, want_g, kind, g, H, energies, h->own_stream, nullptr, fused);
+                if (!fused)
+                    hip_check(hipMemcpyAsync(h->pinned, h->d_scalars.ptr, 16, hipMemcpyDeviceToHost, h->own_stream),
+                              "read-back");
             } catch (...) {
                 hipGraph_t broken = nullptr;
                 (void)hipStreamEndCapture(h->own_stream, &broken);
@@ -438,8 +445,14 @@ double run_eval_once(lpg_tet_energy* h, const double* x, const double* theta, in
         }
         gc.pending = k;
     }
-    enqueue_eval(h, x, theta, want_g, kind, g, H, energies, stream);
-    const double value = finish(h, stream, kind == 1);
+    enqueue_eval(h, x, theta, want_g, kind, g, H, energies, stream, nullptr, fused);
+    double value;
+    if (fused) {
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        value = read_back(h, kind == 1);
+    } else {
+        value = finish(h, stream, kind == 1);
+    }
     collect_timing(h);
     return value;
 }

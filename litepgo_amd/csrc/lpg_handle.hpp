@@ -114,7 +114,8 @@ struct lpg_tet_energy {
         double *g = nullptr, *H = nullptr, *energies = nullptr;
         hipStream_t stream = nullptr;
     } pending_eval;
-    void* pinned = nullptr; // energy + status + counter read-back
+    void* pinned = nullptr;     // energy + status + counter read-back
+    void* pinned_dev = nullptr; // its device address (the last reduction kernel writes it directly), or null
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double time_ms[4] = {0, 0, 0, 0};
@@ -158,8 +159,10 @@ struct EvalScratch {
 // Checks arguments and enqueues one evaluation (no synchronisation).  energies (device) receives one energy per
 // vertex segment (seg_vbeg); null means the scalar slot (plain energies).  ws: the scratch to use instead of the
 // handle's buffers (then nothing is allocated, and the handle must already be on the device).
+// readback: pinned memory (device address) that the last kernel fills with the energy, status and counter, in place
+// of a copy; only for single-mesh handles (null: the caller copies the scalars itself).
 void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g, double* H,
-                  double* energies, hipStream_t stream, const EvalScratch* ws = nullptr);
+                  double* energies, hipStream_t stream, const EvalScratch* ws = nullptr, void* readback = nullptr);
 // Enqueues the read-back of status, counter and the first energy from `scalars` (null: the handle's own slot),
 // synchronises, maps the status to an error.
 double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter, const void* scalars = nullptr);

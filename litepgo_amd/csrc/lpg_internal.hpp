@@ -168,9 +168,11 @@ size_t assemble_lds_bytes(const LaunchConfig& lc, const TileCaps& caps, bool glo
 size_t assemble_lds_limit(const LaunchConfig& lc);
 // Energy per mesh from the per-vertex energies: chunk c sums vertex_E[chunk_begin[c] .. chunk_begin[c+1]),
 // segment m sums chunk_sum[seg_chunk[m] .. seg_chunk[m+1]) into out[m].
+// readback (one segment only; device-accessible pinned memory, or null): receives the energy, the status word and
+// `counter` (16 bytes) from the last kernel of the evaluation, in place of a separate device-to-host copy.
 void launch_energy_reduce(const double* vertex_E, const int32_t* chunk_begin, int num_chunks, double* chunk_sum,
                           const int32_t* seg_chunk, int num_segments, double* out, unsigned* status,
-                          hipStream_t stream);
+                          hipStream_t stream, const unsigned* counter = nullptr, void* readback = nullptr);
 // Chunks of at most 1024 vertices, restarting at every segment start: chunk_begin (chunks + 1 entries) and
 // seg_chunk (segments + 1 entries) for segment starts seg_vbeg (segments + 1 entries, the last = num_vertices).
 void build_energy_chunks(const std::vector<int32_t>& seg_vbeg, std::vector<int32_t>& chunk_begin,

@@ -930,9 +930,13 @@ __global__ __launch_bounds__(256) void k_energy_chunks(const double* __restrict_
         chunk_sum[c] = total;
 }
 
+// readback (one segment only): host-visible pinned memory that receives the energy, the status word and the list
+// counter once every earlier kernel of the evaluation has finished (this is the last one), in place of a separate
+// device-to-host copy.
 __global__ __launch_bounds__(256) void k_energy_segments(const double* __restrict__ chunk_sum,
                                                          const int32_t* __restrict__ seg_chunk,
-                                                         double* __restrict__ out, unsigned* status) {
+                                                         double* __restrict__ out, unsigned* status,
+                                                         const unsigned* counter, void* readback) {
     __shared__ double red[256];
     const int m = blockIdx.x;
     double s = 0.0;
@@ -941,8 +945,19 @@ __global__ __launch_bounds__(256) void k_energy_segments(const double* __restric
     const double total = tree_sum<256>(red, s);
     if (threadIdx.x == 0) {
         out[m] = total;
-        if (!isfinite(total))
+        unsigned st = __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!isfinite(total)) {
             atomicOr(status, kStatusNonFiniteOut);
+            st |= kStatusNonFiniteOut;
+        }
+        if (readback) {
+            double* rb = static_cast<double*>(readback);
+            unsigned* rbu = static_cast<unsigned*>(readback);
+            rb[0] = total;
+            rbu[2] = st;
+            rbu[3] = counter ? __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+            __threadfence_system();
+        }
     }
 }
 
@@ -2025,14 +2040,16 @@ void launch_assemble(const DeviceMesh& m, const EvalArgs& a, const LaunchConfig&
 
 void launch_energy_reduce(const double* vertex_E, const int32_t* chunk_begin, int num_chunks, double* chunk_sum,
                           const int32_t* seg_chunk, int num_segments, double* out, unsigned* status,
-                          hipStream_t stream) {
+                          hipStream_t stream, const unsigned* counter, void* readback) {
+    if (readback && num_segments != 1)
+        fail(kLogic, "launch_energy_reduce: the fused read-back needs exactly one segment");
     if (num_chunks > 0) {
         hipLaunchKernelGGL(k_energy_chunks, dim3(num_chunks), dim3(256), 0, stream, vertex_E, chunk_begin, chunk_sum);
         check_launch("k_energy_chunks");
     }
     if (num_segments > 0) {
         hipLaunchKernelGGL(k_energy_segments, dim3(num_segments), dim3(256), 0, stream, chunk_sum, seg_chunk, out,
-                           status);
+                           status, counter, readback);
         check_launch("k_energy_segments");
     }
 }
