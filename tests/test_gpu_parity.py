@@ -394,3 +394,46 @@ def test_projection_store_grows_when_more_tets_are_listed(oracle_mod):
     dev.set_graph(False)
     u2 = (0.002 * rng.uniform(-1, 1, X.shape)).reshape(-1)
     check_against_oracle(dev, ref, u2)
+
+
+def test_graph_replays_report_energy_status_and_count():
+    """In graph mode a single-mesh evaluation hands energy, status word and list count to the host from its last
+    kernel.  Replays of one captured graph with new contents in the same buffers: the energy and the indefinite count
+    match an eager evaluation of the same state bit for bit, a non-finite x or an inverted element raises the
+    reference's error class, and the handle recovers."""
+    import torch
+
+    w = workloads.plain(6, 1e5, 0.45, 0.05, stretch=0.2, seed=4)
+    dev = VolumetricFemEnergy(w.num_vertices, w.tets, w.rest, w.theta)
+    states = [w.u, w.u * 1.5, 0.5 * w.u]
+    eager = []
+    for u in states:
+        v, g, H = dev.eval_device(torch.from_numpy(u).cuda(), True, "projected")
+        eager.append((v, dev.last_indefinite_count, g.cpu().numpy(), H.cpu().numpy()))
+    assert any(e[1] > 0 for e in eager)
+    x = torch.from_numpy(w.u.copy()).cuda()
+    g = torch.empty(dev.num_dofs, dtype=torch.float64, device="cuda")
+    H = torch.empty(dev.nnz, dtype=torch.float64, device="cuda")
+    dev.set_graph(True)
+    try:
+        for _ in range(2):  # eager, then captured
+            dev.eval_device(x, True, "projected", out_g=g, out_H=H)
+        for rep in range(3):
+            for (v, count, ge, He), u in zip(eager, states):
+                x.copy_(torch.from_numpy(u))
+                got = dev.eval_device(x, True, "projected", out_g=g, out_H=H)[0]
+                assert got == v and dev.last_indefinite_count == count, (rep, got, v)
+                assert np.array_equal(g.cpu().numpy(), ge) and np.array_equal(H.cpu().numpy(), He)
+                torch.cuda.synchronize()
+        bad = w.u.copy()
+        bad[7] = math.nan
+        x.copy_(torch.from_numpy(bad))
+        with pytest.raises(ValueError, match="finite"):
+            dev.eval_device(x, True, "projected", out_g=g, out_H=H)
+        x.copy_(torch.from_numpy(-2.0 * w.rest.reshape(-1)))
+        with pytest.raises(_native.DomainError):
+            dev.eval_device(x, True, "projected", out_g=g, out_H=H)
+        x.copy_(torch.from_numpy(w.u))
+        assert dev.eval_device(x, True, "projected", out_g=g, out_H=H)[0] == eager[0][0]
+    finally:
+        dev.set_graph(False)
