@@ -255,7 +255,7 @@ double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter, const vo
 }
 
 void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g, double* H,
-                  double* energies, hipStream_t stream, const EvalScratch* ws, void* readback) {
+                  double* energies, hipStream_t stream, const EvalScratch* ws, void* readback, bool skip_clear) {
     if (kind < -1 || kind > 1)
         fail(kInvalid, "Energy: invalid Hessian kind");
     if ((want_g && !g && h->setup.num_dofs > 0) || (kind >= 0 && !H && h->setup.nnz > 0) ||
@@ -309,7 +309,10 @@ void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int w
     }
     if (h->timing)
         hip_check(hipEventRecord(h->ev[0], stream), "hipEventRecord");
-    launch_clear(a.status, a.counter, stream);
+    if (!readback)
+        h->status_clean = false;
+    if (!(readback && skip_clear))
+        launch_clear(a.status, a.counter, stream);
     if (kind == 1) {
         launch_project_prepass(m, a, stream);
         launch_project_jacobi(m, a, stream);
@@ -423,7 +426,9 @@ double run_eval_once(lpg_tet_energy* h, const double* x, const double* theta, in
             gc.reset();
             hip_check(hipStreamBeginCapture(h->own_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             try {
-                enqueue_eval(h, x, theta, want_g, kind, g, H, energies, h->own_stream, nullptr, fused);
+                // With the fused read-back the graph holds no clearing kernel: its last kernel leaves the status
+                // word and counter zero, and a replay after anything else clears them first (below).
+                enqueue_eval(h, x, theta, want_g, kind, g, H, energies, h->own_stream, nullptr, fused, fused != nullptr);
                 if (!fused)
                     hip_check(hipMemcpyAsync(h->pinned, h->d_scalars.ptr, 16, hipMemcpyDeviceToHost, h->own_stream),
                               "read-back");
@@ -439,16 +444,23 @@ double run_eval_once(lpg_tet_energy* h, const double* x, const double* theta, in
             gc.key = k;
         }
         if (gc.exec && gc.key == k) {
+            if (fused && !h->status_clean)
+                launch_clear(h->d_status(), h->d_counter(), stream);
+            h->status_clean = false;
             hip_check(hipGraphLaunch(gc.exec, stream), "hipGraphLaunch");
             hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            h->status_clean = fused != nullptr;
             return read_back(h, kind == 1);
         }
         gc.pending = k;
     }
-    enqueue_eval(h, x, theta, want_g, kind, g, H, energies, stream, nullptr, fused);
+    const bool skip_clear = fused && h->status_clean;
+    h->status_clean = false;
+    enqueue_eval(h, x, theta, want_g, kind, g, H, energies, stream, nullptr, fused, skip_clear);
     double value;
     if (fused) {
         hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        h->status_clean = true;
         value = read_back(h, kind == 1);
     } else {
         value = finish(h, stream, kind == 1);
@@ -477,6 +489,7 @@ lpg_status derivative_call(lpg_tet_energy* h, const double* x, const double* the
             fail(kInvalid, "NeoHookean requires finite lambda >= 0 and mu > 0");
         ensure_device(h, snapshot_fp);
         const hipStream_t s = static_cast<hipStream_t>(stream);
+        h->status_clean = false;
         launch_clear(h->d_status(), h->d_counter(), s);
         launch(h->mesh(), h->d_status(), s);
         (void)finish(h, s, false);
@@ -495,6 +508,7 @@ double feasible_step_call(lpg_tet_energy* h, const double* x, const double* dx, 
     if (!st.theta_valid && st.num_tets > 0)
         fail(kInvalid, "NeoHookean requires finite lambda >= 0 and mu > 0");
     ensure_device(h);
+    h->status_clean = false;
     const int segments = per_mesh ? static_cast<int>(h->seg_tbeg.size()) - 1 : 1;
     double* out = alphas ? alphas : h->d_energy();
     launch_feasible_step(h->mesh(), x, dx, per_mesh ? h->d_seg_tbeg.as<int32_t>() : h->d_seg_all.as<int32_t>(),

@@ -114,6 +114,9 @@ struct lpg_tet_energy {
         double *g = nullptr, *H = nullptr, *energies = nullptr;
         hipStream_t stream = nullptr;
     } pending_eval;
+    // The device status word and list counter are known to be zero (the last operation was an evaluation whose last
+    // kernel read and zeroed them): the next single-mesh evaluation then needs no clearing kernel.
+    bool status_clean = false;
     void* pinned = nullptr;     // energy + status + counter read-back
     void* pinned_dev = nullptr; // its device address (the last reduction kernel writes it directly), or null
     bool timing = false;
@@ -161,8 +164,10 @@ struct EvalScratch {
 // handle's buffers (then nothing is allocated, and the handle must already be on the device).
 // readback: pinned memory (device address) that the last kernel fills with the energy, status and counter, in place
 // of a copy; only for single-mesh handles (null: the caller copies the scalars itself).
+// skip_clear: the status word and counter are already zero (readback only; the last kernel zeroes them again).
 void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g, double* H,
-                  double* energies, hipStream_t stream, const EvalScratch* ws = nullptr, void* readback = nullptr);
+                  double* energies, hipStream_t stream, const EvalScratch* ws = nullptr, void* readback = nullptr,
+                  bool skip_clear = false);
 // Enqueues the read-back of status, counter and the first energy from `scalars` (null: the handle's own slot),
 // synchronises, maps the status to an error.
 double finish(lpg_tet_energy* h, hipStream_t stream, bool want_counter, const void* scalars = nullptr);

@@ -951,11 +951,16 @@ __global__ __launch_bounds__(256) void k_energy_segments(const double* __restric
             st |= kStatusNonFiniteOut;
         }
         if (readback) {
+            // The status word and the list counter are zeroed for the next evaluation once read (the host
+            // clears them itself only after an operation that leaves them set).
             double* rb = static_cast<double*>(readback);
             unsigned* rbu = static_cast<unsigned*>(readback);
             rb[0] = total;
             rbu[2] = st;
             rbu[3] = counter ? __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+            __hip_atomic_store(status, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (counter)
+                __hip_atomic_store(const_cast<unsigned*>(counter), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __threadfence_system();
         }
     }

@@ -338,6 +338,7 @@ lpg_status lpg_tet_equilibrium_vjp_device(lpg_tet_energy* h, const uint8_t* fixe
         if (w.n > 0)
             lpg::sensitivity_solve(h, w, fixed_dev, theta_dev, x_dev, w_dev, lam.as<double>(), *opt, s);
         // dL/dtheta = -(df/dtheta)^T lambda (static_equilibrium_response.cpp:70-73).
+        h->status_clean = false;
         lpg::launch_clear(h->d_status(), nullptr, s);
         lpg::launch_vjp(h->mesh(), x_dev, theta_dev, nullptr, lam.as<double>(), nullptr, gtheta_dev, h->d_status(), s);
         (void)lpg::finish(h, s, false);
@@ -363,6 +364,7 @@ lpg_status lpg_tet_equilibrium_jvp_device(lpg_tet_energy* h, const uint8_t* fixe
         // rhs = -(df/dtheta) dtheta (static_equilibrium_response.cpp:66-69).
         lpg::ScopedBuffer rhs;
         rhs.ensure(static_cast<size_t>(w.n) * sizeof(double));
+        h->status_clean = false;
         lpg::launch_clear(h->d_status(), nullptr, s);
         lpg::launch_force_jvp_theta(h->mesh(), x_dev, theta_dev, dtheta_dev, rhs.as<double>(), h->d_status(), s);
         (void)lpg::finish(h, s, false);
