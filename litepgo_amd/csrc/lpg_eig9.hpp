@@ -422,7 +422,11 @@ LPG_EIG_FN bool psd_part(double a[45], double X[4][N], double wts[4], int& ns, b
             y[i] = kStartVec[j][i];
 #pragma unroll
         for (int it = 0; it < 4; ++it) {
-            // Modified Gram-Schmidt against the vectors already found, then normalisation.
+            // Modified Gram-Schmidt against the vectors already found, then normalisation.  The last pass runs
+            // twice: in a repeated eigenvalue the last vector's remaining component can be small, and a single
+            // pass then leaves it measurably non-orthogonal to the others.
+#pragma unroll
+            for (int pass = 0; pass < (it == 3 ? 2 : 1); ++pass)
 #pragma unroll
             for (int jj = 0; jj < j; ++jj) {
                 double dot = 0.0;
