@@ -225,6 +225,9 @@ constexpr int kSpill = (N - 1) * N / 2 - 1;
 template <class Rsqrt, class Recip, class Spill>
 LPG_EIG_FN bool psd_part(double a[45], double X[4][N], double wts[4], int& ns, bool& add_M, double scale,
                          Rsqrt rsqrt, Recip recip, Spill& spill) {
+    // No implicit FMA contraction: the QL sweep has two code paths, taken per wave, that must round alike, so every
+    // lane's result depends on its own matrix only (not on which lanes share its wave).
+#pragma clang fp contract(off)
     double e[N], d[N];
     // ---- Householder reduction; reflector k (unit vector on indices k+1..N-1) goes to the spill store at offset
     // off(k) = sum_{k' < k} (N - 1 - k').
@@ -325,6 +328,33 @@ LPG_EIG_FN bool psd_part(double a[45], double X[4][N], double wts[4], int& ns, b
                     dm = d[i];
             g = dm - d[l] + e[l] * recip(g + (g >= 0.0 ? r : -r));
             double s = 1.0, c = 1.0, p = 0.0;
+            if (!any_lane(active && m != N - 1)) {
+                // No active lane is split below l (the usual case): every e[i], i >= l, is above the deflation
+                // threshold, so f = s e[i] never vanishes and no rotation meets f = g = 0.  The sweep then needs
+                // neither the per-lane start m nor the exact-split branch, and does the same arithmetic as the
+                // general sweep below.
+                if (active) {
+#pragma unroll
+                    for (int i = N - 2; i >= l; --i) {
+                        const double f = s * e[i];
+                        const double b = c * e[i];
+                        const double rr2 = fma(f, f, g * g);
+                        const double ir = rsqrt(rr2);
+                        e[i + 1] = rr2 * ir;
+                        s = f * ir;
+                        c = g * ir;
+                        g = d[i + 1] - p;
+                        const double rq = fma(d[i] - g, s, 2.0 * c * b);
+                        p = s * rq;
+                        d[i + 1] = g + p;
+                        g = fma(c, rq, -b);
+                    }
+                    d[l] -= p;
+                    e[l] = g;
+                    e[N - 1] = 0.0;
+                }
+                continue;
+            }
             bool alive = active;
 #pragma unroll
             for (int i = N - 2; i >= l; --i) {
