@@ -256,12 +256,15 @@ void random_rotation(std::mt19937_64& rng, double R[9]) {
 
 } // namespace
 
+// Usage: eig9_test [fixture] [scale]; scale multiplies the numbers of random, clustered and neo-Hookean cases (a
+// stress run: scale 30).
 int main(int argc, char** argv) {
+    const int scale = argc > 2 ? std::max(1, std::atoi(argv[2])) : 1;
     std::mt19937_64 rng(7);
     std::uniform_real_distribution<double> U(-1, 1);
     int fails = 0, cases = 0;
     // Random symmetric matrices over several scales.
-    for (int t = 0; t < 2000; ++t) {
+    for (int t = 0; t < 2000 * scale; ++t) {
         std::vector<double> A(81);
         const double sc = std::pow(10.0, (t % 9) - 4);
         for (int r = 0; r < N; ++r)
@@ -293,7 +296,7 @@ int main(int argc, char** argv) {
         return Q;
     };
     // Repeated, zero and clustered eigenvalues: Q diag(lambda) Q^T with a random orthogonal Q.
-    for (int t = 0; t < 1200; ++t) {
+    for (int t = 0; t < 1200 * scale; ++t) {
         const std::vector<double> Q = random_basis();
         std::vector<double> A(81, 0.0);
         double lam[N];
@@ -363,7 +366,7 @@ int main(int argc, char** argv) {
     }
     // Reduced neo-Hookean element Hessians (E = 1e5, nu in [0.3, 0.49]) at random deformations, including strongly
     // compressed and stretched states (indefinite tangents).
-    for (int t = 0; t < 3000; ++t) {
+    for (int t = 0; t < 3000 * scale; ++t) {
         double Dm[9], F[9];
         for (int q = 0; q < 9; ++q)
             Dm[q] = (q % 4 == 0 ? 0.05 : 0.0) + 0.01 * U(rng);
