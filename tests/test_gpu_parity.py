@@ -167,6 +167,43 @@ def test_projection_far_from_rest(oracle_mod):
         assert dev.last_indefinite_count == len(tets)
 
 
+def regular_tets_scaled(count, scales, seed):
+    """Disjoint regular tetrahedra (random size, orientation and place), each deformed by F = s R (s from `scales` in
+    turn, R a random rotation).  For a regular tet the reduced element Hessian is a multiple of the tangent in a
+    rotated basis, and at F = s R the tangent's spectrum is {mu + p/s^2 (x3), mu - p/s^2 (x5), mu - p/s^2 + 3 lam/s^2}:
+    exactly repeated eigenvalues."""
+    rng = np.random.default_rng(seed)
+    base = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], float)
+
+    def rotation():
+        Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        return Q if np.linalg.det(Q) > 0 else -Q
+
+    X, U, tets = [], [], []
+    for t in range(count):
+        P = base @ rotation().T * (0.02 * (1.0 + rng.uniform())) + rng.uniform(-1, 1, 3)
+        c = P.mean(0)
+        x = (P - c) @ (scales[t % len(scales)] * rotation()).T + c
+        X.append(P)
+        U.append(x - P)
+        tets.append([4 * t, 4 * t + 2, 4 * t + 1, 4 * t + 3])  # (0, 2, 1, 3): positive orientation of `base`
+    return np.concatenate(X), np.array(tets, dtype=np.int32), np.concatenate(U).reshape(-1)
+
+
+def test_projection_with_exactly_repeated_eigenvalues(oracle_mod):
+    """Element Hessians with exactly repeated eigenvalues (regular tets under F = s R, nu = 0.45): s = 0.7 gives a
+    3-fold negative eigenvalue (its eigenspace is added), s = 1.5 and 2 a 5-fold negative one (the positive part,
+    a 3-fold eigenvalue and a single one, replaces the exact blocks), s = 1 a stress-free rotation (semi-definite
+    with a 3-fold zero eigenvalue: listed, and the clamp returns M itself).  The eigen-clamp's last inverse-iteration
+    vectors inside a repeated eigenvalue are the hard case."""
+    X, tets, u = regular_tets_scaled(96, (0.7, 1.5, 2.0, 1.0), seed=23)
+    lam, mu = lame_parameters(1e5, 0.45)
+    theta = np.tile([lam, mu], (len(tets), 1))
+    dev, ref = make(oracle_mod, X, tets, theta)
+    check_against_oracle(dev, ref, u)
+    assert dev.last_indefinite_count == len(tets)
+
+
 def test_real_mesh_from_veg(oracle_mod):
     """The reference's box-tet.veg asset (243 vertices, 827 irregular tets), read with litepgo_amd.read_veg: the
     state of the reference's own test (u = 1e-3 sin(i)) and a 30-degree twist about the long axis."""
