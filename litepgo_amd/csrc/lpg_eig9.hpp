@@ -309,31 +309,28 @@ LPG_EIG_FN bool psd_part(double a[45], double X[4][N], double wts[4], int& ns, b
 #pragma unroll
     for (int l = 0; l < N - 1; ++l) {
         for (int it = 0; it < 40; ++it) {
-            int m = N - 1;
-#pragma unroll
-            for (int i = N - 2; i >= l; --i)
-                if (fabs(e[i]) <= tiny)
-                    m = i;
-            const bool active = m != l;
+            // A lane is active while e[l] is not negligible, and split when some e[i], l < i < N - 1, is: its
+            // sweep then starts at the first such i (the deflation point m) instead of N - 1.
+            const bool active = !(fabs(e[l]) <= tiny);
             if (!any_lane(active))
                 break;
             if (it == 39)
                 ok = ok && !active;
+            bool split = false;
+#pragma unroll
+            for (int i = N - 2; i > l; --i)
+                split = split || fabs(e[i]) <= tiny;
             double g = (d[l + 1] - d[l]) * (0.5 * recip(e[l]));
             double r = fma(g, g, 1.0) * rsqrt(fma(g, g, 1.0));
-            double dm = d[N - 1];
-#pragma unroll
-            for (int i = N - 2; i >= l; --i)
-                if (i == m)
-                    dm = d[i];
-            g = dm - d[l] + e[l] * recip(g + (g >= 0.0 ? r : -r));
+            const double shift_term = e[l] * recip(g + (g >= 0.0 ? r : -r));
             double s = 1.0, c = 1.0, p = 0.0;
-            if (!any_lane(active && m != N - 1)) {
+            if (!any_lane(active && split)) {
                 // No active lane is split below l (the usual case): every e[i], i >= l, is above the deflation
                 // threshold, so f = s e[i] never vanishes and no rotation meets f = g = 0.  The sweep then needs
                 // neither the per-lane start m nor the exact-split branch, and does the same arithmetic as the
                 // general sweep below.
                 if (active) {
+                    g = d[N - 1] - d[l] + shift_term;
 #pragma unroll
                     for (int i = N - 2; i >= l; --i) {
                         const double f = s * e[i];
@@ -355,6 +352,18 @@ LPG_EIG_FN bool psd_part(double a[45], double X[4][N], double wts[4], int& ns, b
                 }
                 continue;
             }
+            int m = N - 1;
+#pragma unroll
+            for (int i = N - 2; i >= l; --i)
+                if (fabs(e[i]) <= tiny)
+                    m = i;
+            // d[m] is selected by predication (m is per lane).
+            double dm = d[N - 1];
+#pragma unroll
+            for (int i = N - 2; i >= l; --i)
+                if (i == m)
+                    dm = d[i];
+            g = dm - d[l] + shift_term;
             bool alive = active;
 #pragma unroll
             for (int i = N - 2; i >= l; --i) {
