@@ -185,28 +185,35 @@ constexpr int kThreads = 256;
 constexpr int kApplyBlocks = 2048;  // workgroups of k_pcg_apply (at most): p.q partials
 constexpr int kUpdateBlocks = 256;  // workgroups of k_pcg_begin / k_pcg_update (at most): r.z, r.r partials
 
-template <int K> __device__ __forceinline__ void block_reduce(double (&v)[K], double (*red)[kThreads]) {
+// Workgroup sum with the same bits in every thread: an xor butterfly within each wave (partners add the same two
+// values, so all 64 lanes end with identical totals), then the wave totals in wave order.  Two barriers instead of a
+// shared-memory tree's nine.
+template <int K> __device__ __forceinline__ void block_reduce(double (&v)[K], double (*red)[kThreads / kWave]) {
     const int tid = static_cast<int>(threadIdx.x);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1)
+            v[k] += __shfl_xor(v[k], off);
     __syncthreads(); // earlier readers of red are done
+    if (tid % kWave == 0)
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-        red[k][tid] = v[k];
+        for (int k = 0; k < K; ++k)
+            red[k][tid / kWave] = v[k];
     __syncthreads();
-    for (int w = kThreads / 2; w > 0; w >>= 1) {
-        if (tid < w)
 #pragma unroll
-            for (int k = 0; k < K; ++k)
-                red[k][tid] += red[k][tid + w];
-        __syncthreads();
+    for (int k = 0; k < K; ++k) {
+        double t = red[k][0];
+#pragma unroll
+        for (int w = 1; w < kThreads / kWave; ++w)
+            t += red[k][w];
+        v[k] = t;
     }
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-        v[k] = red[k][0];
 }
 
 // This workgroup's sums of K per-thread values into part[k G + block] (G = the grid size).
 template <int K> __device__ __forceinline__ void block_partials(double (&v)[K], double* __restrict__ part) {
-    __shared__ double red[K][kThreads];
+    __shared__ double red[K][kThreads / kWave];
     block_reduce<K>(v, red);
     if (threadIdx.x == 0)
 #pragma unroll
@@ -216,7 +223,7 @@ template <int K> __device__ __forceinline__ void block_partials(double (&v)[K], 
 
 // The total of n partials, the same bits in every thread and workgroup (strided per-thread sums, fixed tree).
 __device__ __forceinline__ double sum_partials(const double* __restrict__ part, int n) {
-    __shared__ double red[1][kThreads];
+    __shared__ double red[1][kThreads / kWave];
     double v[1] = {0.0};
     for (int i = static_cast<int>(threadIdx.x); i < n; i += kThreads)
         v[0] += part[i];
@@ -348,6 +355,48 @@ __global__ __launch_bounds__(kThreads) void k_pcg_apply(const double* __restrict
                                                         double* __restrict__ q, const double* __restrict__ shift,
                                                         double* __restrict__ s, int it, const double* __restrict__ part_u,
                                                         int nu, double* __restrict__ part_a, int num_vertices) {
+    // A group's vertices v, v + G, ... (G groups).  Each vertex's first 3 kLanesPerVertex columns are staged in
+    // registers (its row pointers, then the neighbour ids and values, then z and p_old at those columns); the first
+    // vertex's loads are issued before the r.z partials are summed, which only the combination p = z + beta p_old
+    // waits for.  Wider rows (more than 16 neighbours) take their remaining columns in a plain loop.
+    constexpr int kStaged = 3;
+    const int sub = static_cast<int>(threadIdx.x) % kLanesPerVertex;
+    const int groups = kThreads / kLanesPerVertex * static_cast<int>(gridDim.x);
+    const int first = (blockIdx.x * kThreads + threadIdx.x) / kLanesPerVertex;
+    int nb = 0, cols = 0;
+    int col_j[kStaged];
+    double hv[kStaged][3], zv[kStaged], pov[kStaged], zs[3], ps[3];
+    const auto load = [&](int v) {
+        nb = nbr_ptr[v];
+        cols = 3 * (nbr_ptr[v + 1] - nb);
+        const size_t o = 3 * static_cast<size_t>(v);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            zs[c] = z[o + c];
+            ps[c] = FIRST ? 0.0 : p_old[o + c];
+        }
+        const double* seg = H + 9 * static_cast<size_t>(nb);
+#pragma unroll
+        for (int m = 0; m < kStaged; ++m) {
+            col_j[m] = 0;
+            hv[m][0] = hv[m][1] = hv[m][2] = 0.0;
+            if (cols > 0) { // a vertex in no tet has an empty row
+                int t = sub + kLanesPerVertex * m;
+                t = t < cols ? t : cols - 1; // lanes past the row's end reload its last column, unused
+                col_j[m] = 3 * nbr[nb + t / 3] + t % 3;
+                hv[m][0] = seg[t];
+                hv[m][1] = seg[cols + t];
+                hv[m][2] = seg[2 * cols + t];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < kStaged; ++m) {
+            zv[m] = z[col_j[m]];
+            pov[m] = FIRST ? 0.0 : p_old[col_j[m]];
+        }
+    };
+    if (first < num_vertices)
+        load(first);
     const double rz = sum_partials(part_u, nu);
     double beta = 0.0;
     if (!FIRST) {
@@ -357,20 +406,29 @@ __global__ __launch_bounds__(kThreads) void k_pcg_apply(const double* __restrict
     if (blockIdx.x == 0 && threadIdx.x == 0)
         s[kRz0 + (it & 1)] = rz;
     const double sh = shift ? *shift : 0.0;
-    const int sub = static_cast<int>(threadIdx.x) % kLanesPerVertex;
-    const int groups = kThreads / kLanesPerVertex * static_cast<int>(gridDim.x);
     double acc[1] = {0.0};
-    for (int v = (blockIdx.x * kThreads + threadIdx.x) / kLanesPerVertex; v < num_vertices; v += groups) {
-        const int nb = nbr_ptr[v], deg = nbr_ptr[v + 1] - nb;
-        const int cols = 3 * deg;
-        const double* seg = H + 9 * static_cast<size_t>(nb);
+    for (int v = first; v < num_vertices; v += groups) {
+        if (v != first)
+            load(v);
+        // Columns t = sub, sub + 16, ... in increasing order per lane, then the 16 lanes' sums by a fixed tree.
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-        for (int t = sub; t < cols; t += kLanesPerVertex) {
-            const size_t j = 3 * static_cast<size_t>(nbr[nb + t / 3]) + t % 3;
-            const double pv = FIRST ? z[j] : fma(beta, p_old[j], z[j]);
-            a0 = fma(seg[t], pv, a0);
-            a1 = fma(seg[cols + t], pv, a1);
-            a2 = fma(seg[2 * cols + t], pv, a2);
+#pragma unroll
+        for (int m = 0; m < kStaged; ++m) {
+            const bool in = sub + kLanesPerVertex * m < cols;
+            const double pv = FIRST ? zv[m] : fma(beta, pov[m], zv[m]);
+            a0 = in ? fma(hv[m][0], pv, a0) : a0;
+            a1 = in ? fma(hv[m][1], pv, a1) : a1;
+            a2 = in ? fma(hv[m][2], pv, a2) : a2;
+        }
+        if (cols > kLanesPerVertex * kStaged) {
+            const double* seg = H + 9 * static_cast<size_t>(nb);
+            for (int t = sub + kLanesPerVertex * kStaged; t < cols; t += kLanesPerVertex) {
+                const size_t j = 3 * static_cast<size_t>(nbr[nb + t / 3]) + t % 3;
+                const double pv = FIRST ? z[j] : fma(beta, p_old[j], z[j]);
+                a0 = fma(seg[t], pv, a0);
+                a1 = fma(seg[cols + t], pv, a1);
+                a2 = fma(seg[2 * cols + t], pv, a2);
+            }
         }
 #pragma unroll
         for (int m = kLanesPerVertex / 2; m > 0; m >>= 1) {
@@ -383,7 +441,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_apply(const double* __restrict
             const double av[3] = {a0, a1, a2};
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const double pv = FIRST ? z[o + c] : fma(beta, p_old[o + c], z[o + c]);
+                const double pv = FIRST ? zs[c] : fma(beta, ps[c], zs[c]);
                 const double qv = fixed[o + c] ? 0.0 : fma(sh, pv, av[c]);
                 p[o + c] = pv;
                 q[o + c] = qv;
