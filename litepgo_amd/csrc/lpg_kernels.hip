@@ -903,23 +903,26 @@ __global__ void k_clear(unsigned* status, unsigned* counter) {
 // Energy reduction, independent of the tiling: level 1 sums fixed chunks of at most 1024 per-vertex values
 // (chunk boundaries are relative to the first vertex of each mesh), level 2 sums each mesh's chunk partials.  Both
 // levels use a fixed strided order per lane and a fixed tree, so the energy of a mesh has the same bits whatever the
-// launch shape, and whether the mesh is evaluated alone or inside a batch.
+// launch shape, and whether the mesh is evaluated alone or inside a batch.  The tree: an xor butterfly within each
+// wave (all lanes end with the same bits), then the N / 64 wave totals in wave order (one barrier).
 template <int N> __device__ __forceinline__ double tree_sum(double* red, double s) {
-    red[threadIdx.x] = s;
-    __syncthreads();
 #pragma unroll
-    for (int k = N / 2; k > 0; k >>= 1) {
-        if (static_cast<int>(threadIdx.x) < k)
-            red[threadIdx.x] += red[threadIdx.x + k];
-        __syncthreads();
-    }
-    return red[0];
+    for (int off = kWave / 2; off > 0; off >>= 1)
+        s += __shfl_xor(s, off);
+    if (threadIdx.x % kWave == 0)
+        red[threadIdx.x / kWave] = s;
+    __syncthreads();
+    double t = red[0];
+#pragma unroll
+    for (int w = 1; w < N / kWave; ++w)
+        t += red[w];
+    return t;
 }
 
 __global__ __launch_bounds__(256) void k_energy_chunks(const double* __restrict__ values,
                                                        const int32_t* __restrict__ chunk_begin,
                                                        double* __restrict__ chunk_sum) {
-    __shared__ double red[256];
+    __shared__ double red[256 / kWave];
     const int c = blockIdx.x;
     const int b = chunk_begin[c], e = chunk_begin[c + 1];
     double s = 0.0;
@@ -937,7 +940,7 @@ __global__ __launch_bounds__(256) void k_energy_segments(const double* __restric
                                                          const int32_t* __restrict__ seg_chunk,
                                                          double* __restrict__ out, unsigned* status,
                                                          const unsigned* counter, void* readback) {
-    __shared__ double red[256];
+    __shared__ double red[256 / kWave];
     const int m = blockIdx.x;
     double s = 0.0;
     for (int i = seg_chunk[m] + static_cast<int>(threadIdx.x); i < seg_chunk[m + 1]; i += 256)
@@ -968,7 +971,7 @@ __global__ __launch_bounds__(256) void k_energy_segments(const double* __restric
 
 // The total over segments (a batch evaluated through the single-energy entry points): one workgroup, fixed order.
 __global__ __launch_bounds__(256) void k_energy_total(const double* __restrict__ seg, int n, double* __restrict__ out) {
-    __shared__ double red[256];
+    __shared__ double red[256 / kWave];
     double s = 0.0;
     for (int i = static_cast<int>(threadIdx.x); i < n; i += 256)
         s += seg[i];
