@@ -404,6 +404,32 @@ double run_eval(lpg_tet_energy* h, const double* x, const double* theta, int wan
 
 namespace {
 
+// The fused read-back's status word (pinned bytes 8..11) is set to all ones before a graph replay; the replay's
+// last kernel writes the real status there after the energy and the counter.  Waiting for it to change returns as
+// soon as that kernel has written them, without the stream synchronisation's completion round trip; every earlier
+// kernel of the replay (the gradient and Hessian writers) has finished by then.  A stream that ends without the
+// word changing (a device error) is reported through the stream's own status.
+constexpr unsigned kReadbackArmed = 0xffffffffu;
+
+void arm_readback(lpg_tet_energy* h) {
+    __atomic_store_n(reinterpret_cast<unsigned*>(static_cast<char*>(h->pinned) + 8), kReadbackArmed, __ATOMIC_RELEASE);
+}
+
+void wait_readback(lpg_tet_energy* h, hipStream_t stream) {
+    const unsigned* word = reinterpret_cast<const unsigned*>(static_cast<const char*>(h->pinned) + 8);
+    for (unsigned spin = 1; __atomic_load_n(word, __ATOMIC_ACQUIRE) == kReadbackArmed; ++spin) {
+        if ((spin & 4095u) != 0u)
+            continue;
+        const hipError_t q = hipStreamQuery(stream);
+        if (q == hipErrorNotReady)
+            continue;
+        hip_check(q, "hipStreamQuery");
+        if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == kReadbackArmed)
+            fail(kDevice, "evaluation: the stream finished without writing its read-back");
+        break;
+    }
+}
+
 double run_eval_once(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g,
                      double* H, double* energies, hipStream_t stream) {
     ensure_device(h);
@@ -447,8 +473,13 @@ double run_eval_once(lpg_tet_energy* h, const double* x, const double* theta, in
             if (fused && !h->status_clean)
                 launch_clear(h->d_status(), h->d_counter(), stream);
             h->status_clean = false;
+            if (fused)
+                arm_readback(h);
             hip_check(hipGraphLaunch(gc.exec, stream), "hipGraphLaunch");
-            hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            if (fused)
+                wait_readback(h, stream);
+            else
+                hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
             h->status_clean = fused != nullptr;
             return read_back(h, kind == 1);
         }

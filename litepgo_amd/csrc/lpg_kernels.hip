@@ -958,13 +958,15 @@ __global__ __launch_bounds__(256) void k_energy_segments(const double* __restric
             // clears them itself only after an operation that leaves them set).
             double* rb = static_cast<double*>(readback);
             unsigned* rbu = static_cast<unsigned*>(readback);
+            // The status word goes last, after a system-scope fence: a host that sees it change (the host sets it
+            // to all ones before a replay) also sees the energy and the counter.
             rb[0] = total;
-            rbu[2] = st;
             rbu[3] = counter ? __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
             __hip_atomic_store(status, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (counter)
                 __hip_atomic_store(const_cast<unsigned*>(counter), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __threadfence_system();
+            __hip_atomic_store(&rbu[2], st, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
 }
