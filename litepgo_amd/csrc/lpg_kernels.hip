@@ -2086,9 +2086,10 @@ void launch_project_jacobi(const DeviceMesh& m, const EvalArgs& a, hipStream_t s
     ProjectParams p{m.Xr, m.conn, m.theta, a.x, a.theta, m.num_tets, a.status, a.counter, a.proj_slot, a.proj_list, a.proj_H};
     p.fp = m.fp;
     p.proj_capacity = a.proj_capacity;
-    // One lane per element that could be listed: the list length is only known on the device, so the workgroups
-    // past it exit at once (the listed ones come first).
-    const int blocks = (m.num_tets + 63) / 64;
+    // One lane per store slot, plus the lane just past the store (it reports an overflowing list): the list length
+    // is only known on the device, so the workgroups past it exit at once.
+    const int lanes = std::min<int64_t>(m.num_tets, int64_t{a.proj_capacity} + 1);
+    const int blocks = (lanes + 63) / 64;
     hipLaunchKernelGGL(k_project_jacobi, dim3(blocks), dim3(64), 0, stream, p);
     check_launch("k_project_jacobi");
 }
