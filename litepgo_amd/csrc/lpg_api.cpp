@@ -340,6 +340,8 @@ void enqueue_eval(lpg_tet_energy* h, const double* x, const double* theta, int w
                          segments == 1 ? readback : nullptr);
     if (!energies && segments > 1)
         launch_energy_total(seg_out, segments, a.energy, stream);
+    if (readback && segments != 1)
+        launch_readback(a.energy, a.status, a.counter, readback, stream);
     if (h->timing)
         hip_check(hipEventRecord(h->ev[3], stream), "hipEventRecord");
 }
@@ -433,8 +435,9 @@ void wait_readback(lpg_tet_energy* h, hipStream_t stream) {
 double run_eval_once(lpg_tet_energy* h, const double* x, const double* theta, int want_g, int kind, double* g,
                      double* H, double* energies, hipStream_t stream) {
     ensure_device(h);
-    // A single-mesh evaluation hands its scalars to the host from its last kernel (no copy node).
-    void* fused = h->seg_chunk.size() == 2 ? h->pinned_dev : nullptr;
+    // The evaluation hands its scalars to the host from its last kernel (no copy node): the energy reduction's for a
+    // single mesh, a read-back kernel after it for several.
+    void* fused = h->pinned_dev;
     GraphCache& gc = h->graph;
     if (gc.enabled && !h->timing) {
         GraphKey k;

@@ -199,6 +199,10 @@ void launch_vjp(const DeviceMesh& m, const double* x, const double* theta, const
                 double* gx, double* gtheta, unsigned* status, hipStream_t stream);
 // out[0] = sum of seg[0..n) in a fixed order (one workgroup).
 void launch_energy_total(const double* seg, int n, double* out, hipStream_t stream);
+// Energy, status word and list counter to host-visible pinned memory (status word last), then the status word and
+// the counter zeroed: an evaluation's last kernel when its energy reduction cannot carry the read-back itself.
+void launch_readback(const double* energy, unsigned* status, const unsigned* counter, void* readback,
+                     hipStream_t stream);
 // Per-segment feasible steps (alphas: num_segments doubles on the device, min over the segment's tets, 1 when
 // none limits); also clears the status word.
 void launch_feasible_step(const DeviceMesh& m, const double* x, const double* dx, const int32_t* seg_tbeg,

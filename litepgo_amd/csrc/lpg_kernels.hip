@@ -933,9 +933,25 @@ __global__ __launch_bounds__(256) void k_energy_chunks(const double* __restrict_
         chunk_sum[c] = total;
 }
 
-// readback (one segment only): host-visible pinned memory that receives the energy, the status word and the list
-// counter once every earlier kernel of the evaluation has finished (this is the last one), in place of a separate
-// device-to-host copy.
+// Hands the evaluation's energy, status word and list counter to host-visible pinned memory (readback) from the
+// evaluation's last kernel, in place of a device-to-host copy, and zeroes the status word and the counter for the
+// next evaluation (the host clears them itself only after an operation that leaves them set).  The status word goes
+// last, after a system-scope fence: a host that sees it change (the host sets it to all ones before a replay) also
+// sees the energy and the counter.
+__device__ __forceinline__ void write_readback(double energy, unsigned st, unsigned* status, const unsigned* counter,
+                                               void* readback) {
+    double* rb = static_cast<double*>(readback);
+    unsigned* rbu = static_cast<unsigned*>(readback);
+    rb[0] = energy;
+    rbu[3] = counter ? __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+    __hip_atomic_store(status, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (counter)
+        __hip_atomic_store(const_cast<unsigned*>(counter), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence_system();
+    __hip_atomic_store(&rbu[2], st, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// readback (one segment only): the read-back above from this kernel, the evaluation's last.
 __global__ __launch_bounds__(256) void k_energy_segments(const double* __restrict__ chunk_sum,
                                                          const int32_t* __restrict__ seg_chunk,
                                                          double* __restrict__ out, unsigned* status,
@@ -953,22 +969,17 @@ __global__ __launch_bounds__(256) void k_energy_segments(const double* __restric
             atomicOr(status, kStatusNonFiniteOut);
             st |= kStatusNonFiniteOut;
         }
-        if (readback) {
-            // The status word and the list counter are zeroed for the next evaluation once read (the host
-            // clears them itself only after an operation that leaves them set).
-            double* rb = static_cast<double*>(readback);
-            unsigned* rbu = static_cast<unsigned*>(readback);
-            // The status word goes last, after a system-scope fence: a host that sees it change (the host sets it
-            // to all ones before a replay) also sees the energy and the counter.
-            rb[0] = total;
-            rbu[3] = counter ? __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-            __hip_atomic_store(status, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (counter)
-                __hip_atomic_store(const_cast<unsigned*>(counter), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __threadfence_system();
-            __hip_atomic_store(&rbu[2], st, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-        }
+        if (readback)
+            write_readback(total, st, status, counter, readback);
     }
+}
+
+// The read-back as a kernel of its own (an evaluation of several segments): the energy slot, status and counter
+// as the earlier kernels left them.
+__global__ void k_readback(const double* energy, unsigned* status, const unsigned* counter, void* readback) {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        write_readback(*energy, __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), status,
+                       counter, readback);
 }
 
 // The total over segments (a batch evaluated through the single-energy entry points): one workgroup, fixed order.
@@ -2062,6 +2073,12 @@ void launch_energy_reduce(const double* vertex_E, const int32_t* chunk_begin, in
                            status, counter, readback);
         check_launch("k_energy_segments");
     }
+}
+
+void launch_readback(const double* energy, unsigned* status, const unsigned* counter, void* readback,
+                     hipStream_t stream) {
+    hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, stream, energy, status, counter, readback);
+    check_launch("k_readback");
 }
 
 void launch_energy_total(const double* seg, int n, double* out, hipStream_t stream) {
