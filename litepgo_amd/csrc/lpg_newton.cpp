@@ -51,11 +51,16 @@ struct ScopedBuffer : DeviceBuffer {
     }
 };
 
+// Scalar slots: 0..4 PCG (r.z ring of two, p.q, r.r, breakdown), 5 shift, 6 dot result, 7 second dot, 8 max-norm.
+constexpr int kRr = 3, kShift = 5, kDot = 6, kMax = 8;
+
 // Device work space of one solve (released when the solve returns).
 struct Work {
     int n = 0;
     ScopedBuffer g, H, d, r, z, p, p_old, q, binv, trial, part, pcg_part, scalars;
-    void* host = nullptr; // pinned read-back of 8 doubles
+    void* host = nullptr;     // pinned read-back of 8 doubles
+    double* host_dev = nullptr; // its device-side address (null: copies only)
+    unsigned seq = 0;           // last sequence number of a residual check written to host[6]
     explicit Work(const lpg_tet_energy* h) {
         const auto& s = h->setup;
         n = s.num_dofs;
@@ -68,6 +73,9 @@ struct Work {
         pcg_part.ensure(static_cast<size_t>(pcg_partials()) * sizeof(double));
         scalars.ensure(16 * sizeof(double));
         hip_check(hipHostMalloc(&host, 8 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+        void* dev = nullptr;
+        if (hipHostGetDevicePointer(&dev, host, 0) == hipSuccess)
+            host_dev = static_cast<double*>(dev);
     }
     ~Work() {
         if (host)
@@ -84,10 +92,32 @@ struct Work {
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
         return static_cast<const double*>(host);
     }
+    // r.r and the breakdown flag after the work enqueued so far: k_pcg_finish writes them to host[4..5] and then its
+    // sequence number to host[6]; the host waits for that number instead of a copy and a stream synchronisation.  A
+    // stream that ends without it (a device error) is reported through the stream's status.
+    const double* check_residual(const DeviceMesh& m, const double* part, hipStream_t s) {
+        if (!host_dev) {
+            launch_pcg_finish(m, sc(), part, s);
+            return read(kRr, 2, s);
+        }
+        double* hd = static_cast<double*>(host);
+        const unsigned want = ++seq;
+        launch_pcg_finish(m, sc(), part, s, host_dev + 4, want);
+        const unsigned* word = reinterpret_cast<const unsigned*>(hd + 6);
+        for (unsigned spin = 1; __atomic_load_n(word, __ATOMIC_ACQUIRE) != want; ++spin) {
+            if ((spin & 4095u) != 0u)
+                continue;
+            const hipError_t q = hipStreamQuery(s);
+            if (q == hipErrorNotReady)
+                continue;
+            hip_check(q, "hipStreamQuery");
+            if (__atomic_load_n(word, __ATOMIC_ACQUIRE) != want)
+                fail(kDevice, "pcg: the stream finished without writing its residual");
+            break;
+        }
+        return hd + 4;
+    }
 };
-
-// Scalar slots: 0..4 PCG (r.z ring of two, p.q, r.r, breakdown), 5 shift, 6 dot result, 7 second dot, 8 max-norm.
-constexpr int kRr = 3, kShift = 5, kDot = 6, kMax = 8;
 
 double device_dot(Work& w, const double* a, const double* b, hipStream_t s) {
     launch_dot(a, b, nullptr, nullptr, w.n, w.part.as<double>(), w.sc() + kDot, s);
@@ -131,8 +161,7 @@ PcgOutcome pcg(lpg_tet_energy* h, Work& w, const uint8_t* fixed, const double* g
                           w.binv.as<double>(), it, sc, part, s);
         out.iterations = it;
         if (it % kCheckEvery == 0 || it == max_iterations) {
-            launch_pcg_finish(m, sc, part, s);
-            const double* v = w.read(kRr, 2, s); // r.r, breakdown flag
+            const double* v = w.check_residual(m, part, s); // r.r, breakdown flag
             out.residual = std::sqrt(std::max(0.0, v[0]));
             if (v[1] != 0.0) {
                 out.breakdown = true;

@@ -334,12 +334,21 @@ __global__ __launch_bounds__(kThreads) void k_pcg_begin(const double* __restrict
     block_partials<2>(acc, part);
 }
 
-// s[kRr] = r.r from the partials of the last k_pcg_begin / k_pcg_update (`nu` workgroups).
+// s[kRr] = r.r from the partials of the last k_pcg_begin / k_pcg_update (`nu` workgroups).  With rb (pinned host
+// memory), also rb[0] = r.r, rb[1] = the breakdown flag and then, after a system-scope fence, the 32-bit word at
+// rb + 2 = seq: a host that sees seq there reads both values without a copy or a stream synchronisation.
 __global__ __launch_bounds__(kThreads) void k_pcg_finish(const double* __restrict__ part_u, int nu,
-                                                         double* __restrict__ s) {
+                                                         double* __restrict__ s, double* rb, unsigned seq) {
     const double rr = sum_partials(part_u + nu, nu);
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         s[kRr] = rr;
+        if (rb) {
+            rb[0] = rr;
+            rb[1] = s[kFlag];
+            __threadfence_system();
+            __hip_atomic_store(reinterpret_cast<unsigned*>(rb + 2), seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
 }
 
 // Iteration `it` (from 1): r.z of the current residual from the `nu` partials of part_u (stored by workgroup 0 in
@@ -572,9 +581,10 @@ void launch_pcg_begin(const DeviceMesh& m, const double* g, const uint8_t* fixed
     check_solver_launch("k_pcg_begin");
 }
 
-void launch_pcg_finish(const DeviceMesh& m, double* scalars, const double* part, hipStream_t s) {
+void launch_pcg_finish(const DeviceMesh& m, double* scalars, const double* part, hipStream_t s, double* rb,
+                       unsigned seq) {
     hipLaunchKernelGGL(k_pcg_finish, dim3(1), dim3(kThreads), 0, s, part + kApplyBlocks,
-                       static_cast<int>(update_blocks(m.num_vertices)), scalars);
+                       static_cast<int>(update_blocks(m.num_vertices)), scalars, rb, seq);
     check_solver_launch("k_pcg_finish");
 }
 

@@ -28,7 +28,9 @@ void launch_block_jacobi(const DeviceMesh& m, const double* H, const uint8_t* fi
 void launch_pcg_begin(const DeviceMesh& m, const double* g, const uint8_t* fixed, const double* binv, double* r,
                       double* d, double* z, double* scalars, double* part, hipStream_t s);
 // scalars[3] = r.r of the last begin / update.
-void launch_pcg_finish(const DeviceMesh& m, double* scalars, const double* part, hipStream_t s);
+// With rb (pinned host memory, device-visible): rb[0] = r.r, rb[1] = breakdown flag, then the word at rb + 2 = seq.
+void launch_pcg_finish(const DeviceMesh& m, double* scalars, const double* part, hipStream_t s, double* rb = nullptr,
+                       unsigned seq = 0);
 // Iteration it >= 1: p = z + beta p_old (it = 1: p = z), q = F (H p) + shift F p (partials of p.q).
 void launch_pcg_apply(const DeviceMesh& m, const double* H, const uint8_t* fixed, const double* z, const double* p_old,
                       double* p, double* q, const double* shift, int it, double* scalars, double* part,
