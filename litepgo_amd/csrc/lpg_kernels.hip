@@ -1743,83 +1743,102 @@ __device__ __forceinline__ double triple(const double a[3], const double b[3], c
 // largest magnitude, det(A + t dA) = c0 + c1 t + c2 t^2 + c3 t^3, its stationary points split [0, 1] into monotone
 // pieces, the first piece whose right end is not safely positive is bisected to the last safe t, and 0.9 of it is
 // returned.  A and dA are row-major 3x3 (column j = entries j, 3 + j, 6 + j).  Requires det A > 0.
-__device__ double det_feasible_step(const double A[9], const double dA[9]) {
-    double a[3][3], d[3][3]; // a[j] = column j of A / scale_j
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        double scale = 0.0;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-            scale = fmax(scale, fmax(fabs(A[3 * i + j]), fabs(dA[3 * i + j])));
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            a[j][i] = A[3 * i + j] / scale;
-            d[j][i] = dA[3 * i + j] / scale;
-        }
+// The bisection runs in steps (bisect) so that the caller can stop a lane whose step cannot be the minimum.
+struct DetStep {
+    double c0, c1, c2, c3;
+    double lo = 0.0, hi = 1.0;
+    bool active = false; // bisecting [lo, hi]
+    double result = 1.0; // the step once not active
+    static constexpr double kEps = 2.220446049250313e-16;
+    __device__ double value(double t) const {
+        return fma(fma(fma(c3, t, c2), t, c1), t, c0);
     }
-    const double c0 = triple(a[0], a[1], a[2]);
-    const double c1 = triple(d[0], a[1], a[2]) + triple(a[0], d[1], a[2]) + triple(a[0], a[1], d[2]);
-    const double c2 = triple(d[0], d[1], a[2]) + triple(d[0], a[1], d[2]) + triple(a[0], d[1], d[2]);
-    const double c3 = triple(d[0], d[1], d[2]);
-    if (!(c0 > 0.0))
-        return 0.0;
-    constexpr double kEps = 2.220446049250313e-16;
-    const auto value = [&](double t) { return fma(fma(fma(c3, t, c2), t, c1), t, c0); };
-    const auto tolerance = [&](double t) {
+    __device__ double tolerance(double t) const {
         return 64.0 * kEps * (((fabs(c3) * t + fabs(c2)) * t + fabs(c1)) * t + c0);
-    };
-    // Stationary points of the cubic inside (0, 1), in increasing order (2 marks an absent one); with 0 and 1 they
-    // split [0, 1] into the pieces [0, k1], [k1, k2], [k2, 1].  Registers only (no dynamically indexed array).
-    const auto inside = [](double t) { return t > 0.0 && t < 1.0 ? t : 2.0; };
-    double r1 = 2.0, r2 = 2.0;
-    const double qa = 3.0 * c3, qb = 2.0 * c2, qc = c1;
-    if (qa == 0.0) {
-        if (qb != 0.0)
-            r1 = inside(-qc / qb);
-    } else {
-        double disc = fma(qb, qb, -4.0 * qa * qc);
-        const double err = 8.0 * kEps * (qb * qb + fabs(4.0 * qa * qc));
-        if (disc >= -err) {
-            disc = fmax(0.0, disc);
-            const double q = -0.5 * (qb + copysign(sqrt(disc), qb));
-            if (q == 0.0) {
-                r1 = inside(-qb / (2.0 * qa));
-            } else {
-                r1 = inside(q / qa);
-                r2 = inside(qc / q);
+    }
+    __device__ void setup(const double A[9], const double dA[9]) {
+        double a[3][3], d[3][3]; // a[j] = column j of A / scale_j
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double scale = 0.0;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                scale = fmax(scale, fmax(fabs(A[3 * i + j]), fabs(dA[3 * i + j])));
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                a[j][i] = A[3 * i + j] / scale;
+                d[j][i] = dA[3 * i + j] / scale;
             }
         }
-    }
-    const double k1 = fmin(r1, r2), k2 = fmax(r1, r2);
-    // The first piece whose right end is not safely positive is bisected to the last safe t.
-    double lo = 0.0, hi = 1.0;
-    bool found = false;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const double end = i == 0 ? k1 : (i == 1 ? k2 : 1.0);
-        if (found || end > 1.0)
-            continue;
-        if (value(end) > tolerance(end)) {
-            lo = end;
+        c0 = triple(a[0], a[1], a[2]);
+        c1 = triple(d[0], a[1], a[2]) + triple(a[0], d[1], a[2]) + triple(a[0], a[1], d[2]);
+        c2 = triple(d[0], d[1], a[2]) + triple(d[0], a[1], d[2]) + triple(a[0], d[1], d[2]);
+        c3 = triple(d[0], d[1], d[2]);
+        if (!(c0 > 0.0)) {
+            result = 0.0;
+            return;
+        }
+        // Stationary points of the cubic inside (0, 1), in increasing order (2 marks an absent one); with 0 and 1
+        // they split [0, 1] into the pieces [0, k1], [k1, k2], [k2, 1].  Registers only (no dynamically indexed
+        // array).
+        const auto inside = [](double t) { return t > 0.0 && t < 1.0 ? t : 2.0; };
+        double r1 = 2.0, r2 = 2.0;
+        const double qa = 3.0 * c3, qb = 2.0 * c2, qc = c1;
+        if (qa == 0.0) {
+            if (qb != 0.0)
+                r1 = inside(-qc / qb);
         } else {
-            hi = end;
-            found = true;
+            double disc = fma(qb, qb, -4.0 * qa * qc);
+            const double err = 8.0 * kEps * (qb * qb + fabs(4.0 * qa * qc));
+            if (disc >= -err) {
+                disc = fmax(0.0, disc);
+                const double q = -0.5 * (qb + copysign(sqrt(disc), qb));
+                if (q == 0.0) {
+                    r1 = inside(-qb / (2.0 * qa));
+                } else {
+                    r1 = inside(q / qa);
+                    r2 = inside(qc / q);
+                }
+            }
+        }
+        const double k1 = fmin(r1, r2), k2 = fmax(r1, r2);
+        // The first piece whose right end is not safely positive is bisected to the last safe t.
+        bool found = false;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double end = i == 0 ? k1 : (i == 1 ? k2 : 1.0);
+            if (found || end > 1.0)
+                continue;
+            if (value(end) > tolerance(end)) {
+                lo = end;
+            } else {
+                hi = end;
+                found = true;
+            }
+        }
+        active = found;
+    }
+    // Up to `steps` halvings (all of them for steps < 0); the bracket is final once the midpoint equals an end (at
+    // most ~1100 halvings for doubles in [0, 1]), and the step is then 0.9 lo.
+    __device__ void bisect(int steps) {
+        for (int k = 0; active && k != steps; ++k) {
+            const double mid = lo + (hi - lo) / 2.0;
+            if (mid == lo || mid == hi) {
+                active = false;
+                result = 0.9 * lo;
+                break;
+            }
+            if (value(mid) > tolerance(mid))
+                lo = mid;
+            else
+                hi = mid;
         }
     }
-    if (!found)
-        return 1.0;
-    // At most ~1100 halvings reach mid == lo or mid == hi for doubles in [0, 1].
-    for (;;) {
-        const double mid = lo + (hi - lo) / 2.0;
-        if (mid == lo || mid == hi)
-            break;
-        if (value(mid) > tolerance(mid))
-            lo = mid;
-        else
-            hi = mid;
+    // An upper bound of the final step: 0.9 hi while bisecting (lo stays below hi), else the step itself.
+    __device__ double upper() const {
+        return active ? 0.9 * hi : result;
     }
-    return 0.9 * lo;
-}
+};
 
 struct StepParams {
     const double* Xr;
@@ -1843,11 +1862,15 @@ __global__ void k_step_init(unsigned long long* alpha_bits, int n, unsigned* sta
         status[0] = 0u;
 }
 
+// Bisection steps after which a wave publishes an upper bound of its minimum and drops the lanes that cannot reach
+// the segment's minimum (see the kernel).
+constexpr int kStepChecks[2] = {4, 8};
+
 __global__ __launch_bounds__(256) void k_feasible_step(StepParams p) {
     const int e = blockIdx.x * 256 + threadIdx.x;
-    double alpha = 1.0;
     int seg = 0;
     unsigned flags = 0;
+    DetStep ds;
     // Every entry of x and dx is checked (grid stride), also those of vertices that no tet uses.
     for (int i = e; i < p.num_dofs; i += 256 * gridDim.x)
         if (!(isfinite(p.x[i]) && isfinite(p.dx[i])))
@@ -1887,7 +1910,7 @@ __global__ __launch_bounds__(256) void k_feasible_step(StepParams p) {
         if (!fin)
             flags |= kStatusNonFiniteStep;
         if (!flags)
-            alpha = det_feasible_step(k.F, dF);
+            ds.setup(k.F, dF);
         int lo = 0, hi = p.num_segments; // segment of tet e: seg_tbeg[seg] <= e < seg_tbeg[seg + 1]
         while (hi - lo > 1) {
             const int mid = (lo + hi) / 2;
@@ -1901,6 +1924,40 @@ __global__ __launch_bounds__(256) void k_feasible_step(StepParams p) {
     // Wave minimum when the whole wave lies in one segment (one atomic), else one atomic per lane.
     const int seg0 = __shfl(seg, 0);
     const bool uniform = __all(seg == seg0 || e >= p.num_tets);
+    // Pruning, which leaves the minimum's bits unchanged.  After a few halvings each lane holds an upper bound of its
+    // step (0.9 hi, or the step itself once known); the wave publishes the least of them to the segment's word and
+    // reads the word back (B, the least upper bound published so far, not below the segment's minimum).  A lane
+    // whose step is at least 0.9 lo > B cannot be the minimum and stops (contributing 1, which changes no minimum);
+    // the lane attaining the minimum m has 0.9 lo <= m <= B and continues.  So the words end at the same minimum.
+    // Lanes past the last tet and lanes with an error flag contribute 1 (the host raises on the flag).
+    const bool lane_ok = e < p.num_tets && !flags;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        ds.bisect(c == 0 ? kStepChecks[0] : kStepChecks[1] - kStepChecks[0]);
+        if (!__any(ds.active))
+            break;
+        double ub = lane_ok ? ds.upper() : 1.0;
+        unsigned long long* word = p.alpha_bits + (uniform ? seg0 : seg);
+        if (uniform) {
+#pragma unroll
+            for (int off = kWave / 2; off > 0; off >>= 1)
+                ub = fmin(ub, __shfl_xor(ub, off));
+        }
+        // Read first, and publish only a bound that lowers the word: every wave's atomic on one L2 address would
+        // serialize.
+        double bound = __longlong_as_double(
+            static_cast<long long>(__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+        if ((!uniform || (threadIdx.x % kWave) == 0) && (uniform || e < p.num_tets) && ub < bound)
+            atomicMin(word, static_cast<unsigned long long>(__double_as_longlong(ub)));
+        if (uniform)
+            bound = fmin(bound, ub); // the wave's own least upper bound counts too
+        if (ds.active && 0.9 * ds.lo > bound) {
+            ds.active = false;
+            ds.result = 1.0;
+        }
+    }
+    ds.bisect(-1);
+    const double alpha = lane_ok ? ds.result : 1.0;
     if (uniform) {
         double m = alpha;
 #pragma unroll
