@@ -54,7 +54,10 @@ lpg_status lpg_tet_batch_max_feasible_step_device(lpg_tet_energy* energy, const 
 
 /* Graph mode (any handle): the first device evaluation with a given set of buffers, kind and launch plan runs
  * eagerly; a repeat of it is captured into a hipGraph (kernels plus the status read-back) and later repeats
- * replay the graph, one launch per evaluation.  Changing buffers, kind or plan falls back to the eager path. */
+ * replay the graph, one launch per evaluation.  Changing buffers, kind or plan falls back to the eager path.  A
+ * replay returns as soon as its last kernel has written the energy, status word and list count to the handle's
+ * pinned memory, without synchronising the stream: every kernel that writes the gradient or Hessian values has
+ * finished by then, and later work on `stream` stays ordered after the replay. */
 lpg_status lpg_tet_energy_set_graph(lpg_tet_energy* energy, int32_t enable);
 
 /* Autotuner: times the assembly on this handle's own data (x_dev, the given kind) for each candidate launch
