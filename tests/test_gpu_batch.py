@@ -94,6 +94,35 @@ def test_graph_replays_match_eager(torch, meshes):
     assert torch.equal(E2, E0) and torch.equal(g2, g0) and torch.equal(H2, H0)
 
 
+def test_graph_replays_report_status_and_recover(torch, meshes):
+    """In graph mode a batch's replay hands energy and status to the host from a read-back kernel: a non-finite x
+    raises the reference's error class from a replay, and the next replay with good contents is exact again."""
+    import math
+
+    batch = BatchedFemEnergy(meshes)
+    x = union_x(torch, meshes)
+    E0, g0, H0 = (t.clone() for t in batch.eval_device(x, True, "projected"))
+    E = torch.empty_like(E0)
+    g = torch.empty_like(g0)
+    H = torch.empty_like(H0)
+    xr = x.clone()
+    batch.set_graph(True)
+    try:
+        for _ in range(2):  # eager, then captured
+            batch.eval_device(xr, True, "projected", out_E=E, out_g=g, out_H=H)
+        for r in range(3):
+            xr.copy_(x)
+            xr[5] = math.nan
+            with pytest.raises(ValueError, match="finite"):
+                batch.eval_device(xr, True, "projected", out_E=E, out_g=g, out_H=H)
+            xr.copy_(x)
+            batch.eval_device(xr, True, "projected", out_E=E, out_g=g, out_H=H)
+            torch.cuda.synchronize()
+            assert torch.equal(E, E0) and torch.equal(g, g0) and torch.equal(H, H0), r
+    finally:
+        batch.set_graph(False)
+
+
 def test_autotune_keeps_results_and_caches_its_choice(torch, meshes):
     """The autotuner's choice changes no bit; a second handle with the same mesh-size histogram takes the cached
     choice without timing, and a negative repeat count times again."""

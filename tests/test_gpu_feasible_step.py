@@ -72,6 +72,28 @@ def test_grid_random_directions_match_oracle(torch, oracle_mod, seed):
             assert np.isfinite(dev.eval_value(u + a * dx))
 
 
+def test_equal_steps_everywhere_match_oracle(torch, oracle_mod):
+    """A uniform compression along x of a 12^3 grid gives every tet the same step: thousands of lanes tie for the
+    minimum, so the kernel's pruning (lanes whose step cannot reach the segment's least published bound stop
+    bisecting) must keep a lane that attains it.  F = diag(1 - c t, 1, 1), so det F has the simple root t = 1 / c
+    and the step is 0.9 / c up to the reference's 64 eps tolerance; the result equals the oracle's."""
+    X, tets = kuhn_grid(12)
+    lam, mu = lame_parameters(1e5, 0.4)
+    theta = np.tile([lam, mu], (tets.shape[0], 1))
+    dev = VolumetricFemEnergy(len(X), tets, X, theta)
+    ref = oracle_mod.OracleEnergy(len(X), tets, X, theta)
+    zero = np.zeros(X.size)
+    for c in (1.25, 2.0, 4.0):
+        dx = np.zeros_like(X)
+        dx[:, 0] = -c * X[:, 0]
+        dx = dx.reshape(-1)
+        a = dev.max_feasible_step(zero, dx)
+        r = ref.max_feasible_step(zero, dx)
+        print(f"compression {c}: device {a!r} oracle {r!r}", flush=True)
+        assert abs(a - r) <= TOL * r
+        assert abs(a - 0.9 / c) <= 1e-12 / c
+
+
 def test_full_size_step_property(torch):
     w = workloads.plain_large()
     dev = VolumetricFemEnergy(w.num_vertices, w.tets, w.rest, w.theta)
